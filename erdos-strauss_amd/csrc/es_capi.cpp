@@ -1,0 +1,230 @@
+// es_capi.cpp -- the extern "C" boundary over es::Scanner (include/es_sieve.h).
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/es_sieve.h"
+#include "es_scan.h"
+#include "es_tail.h"
+
+struct es_scanner {
+  es::Scanner sc;
+  es::ScanTiming last;
+};
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+void es_config_default(es_config *c) {
+  if (!c) return;
+  c->wheel_M = 2042040;
+  c->mmax = 10000;
+  c->sieve_P = 0;
+  c->filter_file = nullptr;
+  c->threads = 0;
+  c->device = 0;
+}
+
+const char *es_last_error(void) { return g_err.c_str(); }
+
+int es_scanner_open(const es_config *cfg, es_scanner **out) {
+  if (!cfg || !out) return fail(ES_ERR_ARG, "null argument");
+  *out = nullptr;
+  es_scanner *s = new es_scanner;
+  std::string msg;
+  if (s->sc.init_wheel(cfg->wheel_M, cfg->mmax, msg) ||
+      s->sc.init_filters(cfg->filter_file, cfg->sieve_P, msg)) {
+    delete s;
+    return fail(ES_ERR_CONFIG, msg);
+  }
+  s->sc.threads = cfg->threads;
+  if (cfg->device >= 0 && s->sc.open_device(cfg->device, msg)) {
+    delete s;
+    return fail(ES_ERR_DEVICE, msg);
+  }
+  *out = s;
+  return ES_OK;
+}
+
+void es_scanner_close(es_scanner *s) { delete s; }
+
+int es_check_interval(const es_scanner *s, uint64_t lo, uint64_t hi) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  std::string msg;
+  if (es::check_interval(s->sc.plan(), lo, hi, msg)) return fail(ES_ERR_CONFIG, msg);
+  return ES_OK;
+}
+
+static int run_windows(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin,
+                       std::vector<es::WindowRecord> &recs) {
+  if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
+  std::vector<es::Interval> w(nwin);
+  std::string msg;
+  for (size_t i = 0; i < nwin; ++i) {
+    w[i] = es::Interval{lo[i], hi[i]};
+    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg))
+      return fail(ES_ERR_CONFIG, msg);
+  }
+  if (s->sc.scan(w, recs, s->last, msg)) return fail(ES_ERR_DEVICE, msg);
+  return ES_OK;
+}
+
+int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, int emit_survivors,
+                    char **text, size_t *text_len, size_t *rec_len, int *exit_code) {
+  if (!text) return fail(ES_ERR_ARG, "null text");
+  *text = nullptr;
+  std::vector<es::WindowRecord> recs;
+  const int rc = run_windows(s, lo, hi, nwin, recs);
+  if (rc) return rc;
+  std::string all;
+  for (size_t i = 0; i < nwin; ++i) {
+    const std::string r = s->sc.record_text(recs[i], emit_survivors != 0);
+    if (rec_len) rec_len[i] = r.size();
+    if (exit_code) exit_code[i] = recs[i].survivors ? 2 : 0;
+    all += r;
+  }
+  char *buf = (char *)malloc(all.size() + 1);
+  if (!buf) return fail(ES_ERR_ARG, "out of host memory");
+  memcpy(buf, all.data(), all.size());
+  buf[all.size()] = 0;
+  *text = buf;
+  if (text_len) *text_len = all.size();
+  return ES_OK;
+}
+
+int es_scan_counts(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, es_counts *out) {
+  if (!out && nwin) return fail(ES_ERR_ARG, "null output");
+  std::vector<es::WindowRecord> recs;
+  const int rc = run_windows(s, lo, hi, nwin, recs);
+  if (rc) return rc;
+  for (size_t i = 0; i < nwin; ++i) {
+    const es::WindowRecord &r = recs[i];
+    out[i] = es_counts{r.positions, r.covered, r.sieved, r.mr, r.composite, r.rung, r.direct, r.survivors};
+  }
+  return ES_OK;
+}
+
+int es_survivors(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t *p, uint8_t *is_prime, size_t cap,
+                 size_t *n, es_counts *counts) {
+  if (!s || !n) return fail(ES_ERR_ARG, "null argument");
+  std::vector<es::Interval> w{es::Interval{lo, hi}};
+  std::vector<es::u64> sp;
+  std::vector<uint8_t> fl;
+  std::vector<es::WindowRecord> recs;
+  std::string msg;
+  if (hi > lo && es::check_interval(s->sc.plan(), lo, hi, msg)) return fail(ES_ERR_CONFIG, msg);
+  if (s->sc.survivors(w, recs, sp, fl, s->last, msg)) return fail(ES_ERR_DEVICE, msg);
+  *n = sp.size();
+  for (size_t i = 0; i < sp.size() && i < cap; ++i) {
+    if (p) p[i] = sp[i];
+    if (is_prime) is_prime[i] = fl[i];
+  }
+  if (counts) {
+    const es::WindowRecord &r = recs[0];
+    *counts = es_counts{r.positions, r.covered, r.sieved, r.mr, r.composite, r.rung, r.direct, r.survivors};
+  }
+  return ES_OK;
+}
+
+int es_primality(es_scanner *s, const uint64_t *vals, size_t n, uint8_t *flags) {
+  if (!s || (n && (!vals || !flags))) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.device().primality(vals, n, flags)) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  return ES_OK;
+}
+
+int es_certify(uint64_t p, char *buf, size_t cap) {
+  std::string line;
+  const int kind = es::certify_prime(p, line);
+  if (!buf || line.size() + 1 > cap) return fail(ES_ERR_ARG, "buffer too small");
+  memcpy(buf, line.c_str(), line.size() + 1);
+  return kind;
+}
+
+int es_set_layout(es_scanner *s, uint64_t Mp) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  if (Mp && s->sc.plan().M % Mp) return fail(ES_ERR_ARG, "layout modulus must divide the wheel modulus");
+  s->sc.tuning().fixed_Mp = Mp;
+  return ES_OK;
+}
+
+int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bits) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  if (block == 0 || block > 1024 || block % 64 || max_bits == 0 || max_bits % 32 || max_bits > (1u << 18))
+    return fail(ES_ERR_ARG, "bad launch shape");
+  es::LaunchCfg &c = s->sc.tuning().launch;
+  c.persistent = persistent != 0;
+  c.block = block;
+  c.max_bits = max_bits;
+  return ES_OK;
+}
+
+int es_set_tuning(es_scanner *s, double c_pair, double c_mark, double c_bit) {
+  if (!s || c_pair < 0 || c_mark < 0 || c_bit < 0) return fail(ES_ERR_ARG, "bad tuning");
+  es::Tuning &t = s->sc.tuning();
+  t.c_pair = c_pair;
+  t.c_mark = c_mark;
+  t.c_bit = c_bit;
+  return ES_OK;
+}
+
+uint64_t es_choose_layout(const es_scanner *s, uint64_t w) {
+  return s ? s->sc.choose_layout(w) : 0;
+}
+
+int es_last_timing(const es_scanner *s, es_timing *t) {
+  if (!s || !t) return fail(ES_ERR_ARG, "null argument");
+  const es::ScanTiming &m = s->last;
+  *t = es_timing{m.plan_ms, m.batch_ms, m.device_ms, m.sieve_ms, m.tail_ms, m.total_ms,
+                 m.tiles, m.survivors, m.primes, m.replays};
+  return ES_OK;
+}
+
+int es_wheel_info(const es_scanner *s, uint64_t *lanes, uint64_t *pmin, uint64_t *nf, uint64_t *ncov) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  const es::Plan &p = s->sc.plan();
+  if (lanes) *lanes = p.wheel.classes.size();
+  if (pmin) *pmin = p.wheel.max_pmin;
+  if (nf) *nf = p.filters.mod.size();
+  if (ncov) *ncov = p.filters.ncov;
+  return ES_OK;
+}
+
+int es_mark_count(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t Mp, uint64_t *marks, uint64_t *bits,
+                  uint64_t *pairs, uint64_t *tiles) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  if (Mp && s->sc.plan().M % Mp) return fail(ES_ERR_ARG, "layout modulus must divide the wheel modulus");
+  es::u64 a, b, c, d;
+  std::string msg;
+  if (s->sc.mark_count(lo, hi, Mp, a, b, c, d, msg)) return fail(ES_ERR_DEVICE, msg);
+  if (marks) *marks = a;
+  if (bits) *bits = b;
+  if (pairs) *pairs = c;
+  if (tiles) *tiles = d;
+  return ES_OK;
+}
+
+int es_lds_peak(es_scanner *s, double *ops_per_s) {
+  if (!s || !ops_per_s) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.device().lds_peak(*ops_per_s)) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  return ES_OK;
+}
+
+int es_wheel_classes(const es_scanner *s, uint64_t *out, size_t cap, size_t *n) {
+  if (!s || !n) return fail(ES_ERR_ARG, "null argument");
+  const std::vector<es::u64> &c = s->sc.plan().wheel.classes;
+  *n = c.size();
+  for (size_t i = 0; i < c.size() && i < cap && out; ++i) out[i] = c[i];
+  return ES_OK;
+}
+
+void es_free(void *p) { free(p); }
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// es_device.h -- the device side of the funnel, seen from host C++: upload layouts,
+// run a batch of tiles through marking + compaction + Miller-Rabin, read results back.
+// Only es_device.hip includes HIP headers; everything here is plain C++.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "es_plan.h"
+
+namespace es {
+
+// One unit of device work: bits [k0, k0 + nbits) of one lane of one layout, for one
+// window of the batch. p = lanes[lane] + Mp * k for bit k.
+struct TileDesc {
+  u64 k0;
+  u32 nbits;
+  u32 lane;
+  u32 win;
+  u32 layout;
+};
+static_assert(sizeof(TileDesc) == 24, "TileDesc is shared with the device");
+
+// Per-window device counters (sums over the window's tiles).
+struct WinCounts {
+  u64 positions;      // wheel positions in the window
+  u64 after_covers;   // still alive after stage B
+  u64 survivors;      // alive after stage B' (== after_covers without a sieve)
+};
+
+struct BatchResult {
+  std::vector<WinCounts> counts;      // [nwin]
+  std::vector<u64> surv_p;            // stage-C survivors (any order)
+  std::vector<u32> surv_w;            // their window index
+  std::vector<uint8_t> surv_prime;    // 1 if prime (deterministic MR on the device)
+  double device_ms = 0;               // sieve + MR kernels, HIP events
+  double sieve_ms = 0;                // sieve kernel alone
+  int replays = 0;                    // survivor-buffer overflows that forced a rerun
+};
+
+struct LaunchCfg {
+  bool persistent = true;   // work-queue persistent kernel vs one workgroup per tile
+  bool graph = false;       // replay a captured hipGraph (persistent form only)
+  u32 block = 256;          // threads per workgroup
+  u32 max_bits = 65536;     // largest tile (LDS bitmap = max_bits/8 bytes)
+};
+
+class DeviceCtx {
+ public:
+  DeviceCtx();
+  ~DeviceCtx();
+  DeviceCtx(const DeviceCtx &) = delete;
+  DeviceCtx &operator=(const DeviceCtx &) = delete;
+
+  // Selects the device and creates the stream. 0 or -1 with last_error().
+  int open(int device);
+  // Uploads a layout's tables; returns its id (>= 0) or -1.
+  int add_layout(const Layout &L);
+  // Runs one batch. Tiles must reference valid layouts/lanes, nbits <= cfg.max_bits.
+  int run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
+          BatchResult &out);
+  // Deterministic MR of arbitrary values on the device (parity checks).
+  int primality(const u64 *vals, size_t n, uint8_t *flags);
+  // Measured LDS atomic-AND rate of the whole device (32-bit operations per second).
+  int lds_peak(double &ops_per_s);
+  const std::string &last_error() const { return err_; }
+  int device() const { return device_; }
+  size_t layout_bytes() const { return layout_bytes_; }
+  std::string device_name() const;
+  int cu_count() const { return cus_; }
+
+ private:
+  struct Impl;
+  Impl *impl_ = nullptr;
+  std::string err_;
+  int device_ = -1;
+  int cus_ = 0;
+  size_t layout_bytes_ = 0;
+};
+
+}  // namespace es

@@ -1,0 +1,615 @@
+// es_device.hip -- gfx950 kernels of the verification-sieve funnel and the host code
+// that drives them.
+//
+//   k_sieve   one tile = one LDS bitmap of <= max_bits positions of one lane. Init ANDs
+//             the lane's periodic wheel mask (sub-wheel layouts) and the tail mask, and
+//             applies strides < 32 as whole-word patterns (each word owned by one
+//             thread, no atomics). Larger strides clear bits with LDS atomic AND. Stage
+//             B (certified filters) and stage B' (small primes) are separate phases so
+//             the popcount between them splits `covered` from `sieved` exactly
+//             (rung_scan3.cpp:755-772). Survivors are counted per thread, scanned
+//             across the wave with shuffles and across the workgroup in LDS, placed
+//             with one global atomic per tile, and written as 64-bit integers.
+//             Persistent form: workgroups pull tiles from a device queue.
+//   k_mr      deterministic Miller-Rabin (first twelve prime bases) per survivor, in
+//             64-bit Montgomery arithmetic built from __umul64hi.
+//
+// Reference behaviour: sieve/cuda/mark.cu:22-91 (k_mark), survivors.cu:20-133
+// (k_scan_counts, k_compact, k_primality), rung_scan3.cpp:711-784 (sweep_lane).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "es_device.h"
+
+namespace es {
+namespace dev {
+
+typedef unsigned char u8;
+constexpr int kMaxWaves = 16;   // up to 1024 threads per workgroup
+
+struct LayoutDev {
+  u64 Mp;
+  const u64 *lanes;
+  const u32 *mask;
+  const u32 *step;
+  const u32 *magic;
+  const u32 *c32;
+  const u16 *off;
+  u32 P, Pmagic, Pc32, nl, nf, ncov, cov_small, pr_small, mask_words, pad;
+};
+
+struct BatchHdr {
+  u32 ntiles, nwin, has_sieve, pad;
+};
+
+// x mod s for s < 2^16 with mg = floor(2^32 / s) (2^32 - 1 for s = 1): the quotient
+// estimate is low by at most one, so one conditional subtract finishes.
+__device__ __forceinline__ u32 mod32(u32 x, u32 s, u32 mg) {
+  const u32 q = __umulhi(x, mg);
+  const u32 r = x - q * s;
+  return r >= s ? r - s : r;
+}
+
+// x mod s for 64-bit x: (hi mod s) * (2^32 mod s) + (lo mod s), reduced; every
+// intermediate stays below 2^32 because s < 2^16.
+__device__ __forceinline__ u32 mod64(u64 x, u32 s, u32 mg, u32 c32) {
+  const u32 h = mod32((u32)(x >> 32), s, mg);
+  const u32 l = mod32((u32)x, s, mg);
+  const u32 t = mod32(h * c32, s, mg) + l;
+  return t >= s ? t - s : t;
+}
+
+// Bits 0, s, 2s, ... of a 32-bit word (s < 32).
+__device__ __forceinline__ u32 stride_pattern(u32 s) {
+  u32 p = 0;
+  for (u32 i = 0; i < 32; i += s) p |= 1u << i;
+  return p;
+}
+
+// Clears, in the words this thread owns (w = tid, tid + bs, ...), every bit of the
+// progression k == o (mod s), s < 32, for ordered filters [from, to).
+__device__ __forceinline__ void apply_small(u32 *bits, u32 nw, const LayoutDev *L, const u16 *offrow,
+                                            u64 k0, u32 from, u32 to, u32 tid, u32 bs) {
+  for (u32 j = from; j < to; ++j) {
+    const u32 o = offrow[j];
+    if (o == 0xFFFFu) continue;
+    const u32 s = L->step[j], mg = L->magic[j];
+    const u32 km = mod64(k0, s, mg, L->c32[j]);
+    const u32 base = o >= km ? o - km : o + s - km;        // bit index of the first hit, mod s
+    const u32 d0 = mod32(32u * tid, s, mg);
+    const u32 dstep = mod32(32u * bs, s, mg);
+    u32 r = base >= d0 ? base - d0 : base + s - d0;         // first hit inside word tid
+    const u32 pat = stride_pattern(s);
+    for (u32 w = tid; w < nw; w += bs) {
+      bits[w] &= ~(pat << r);
+      r = r >= dstep ? r - dstep : r + s - dstep;
+    }
+  }
+}
+
+// Clears k == o (mod s) for ordered filters [from, to) with one thread per filter;
+// consecutive rounds reverse the thread order so sorted strides balance.
+__device__ __forceinline__ void apply_scatter(u32 *bits, u32 nb, const LayoutDev *L, const u16 *offrow,
+                                              u64 k0, u32 from, u32 to, u32 tid, u32 bs) {
+  u32 round = 0;
+  for (u32 base = from; base < to; base += bs, ++round) {
+    const u32 j = base + ((round & 1) ? bs - 1 - tid : tid);
+    if (j >= to) continue;
+    const u32 o = offrow[j];
+    if (o == 0xFFFFu) continue;
+    const u32 s = L->step[j];
+    const u32 km = mod64(k0, s, L->magic[j], L->c32[j]);
+    u32 i = o >= km ? o - km : o + s - km;
+    for (; i < nb; i += s) atomicAnd(&bits[i >> 5], ~(1u << (i & 31)));
+  }
+}
+
+__device__ __forceinline__ u32 wave_sum(u32 v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <bool kPersistent>
+__global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
+                                                const BatchHdr *__restrict__ hdr,
+                                                const LayoutDev *__restrict__ lays,
+                                                u64 *__restrict__ counts, u64 *__restrict__ surv_p,
+                                                u32 *__restrict__ surv_w, u32 cap,
+                                                u32 *__restrict__ surv_count, u32 *__restrict__ queue) {
+  extern __shared__ u32 bits[];
+  __shared__ u32 s_tile, s_base;
+  __shared__ u32 s_wpos[kMaxWaves], s_waft[kMaxWaves], s_wsur[kMaxWaves];
+  const u32 tid = threadIdx.x, bs = blockDim.x;
+  const u32 lane64 = tid & 63u, wave = tid >> 6, nwaves = (bs + 63u) >> 6;
+  const u32 ntiles = hdr->ntiles;
+  const bool sieve = hdr->has_sieve != 0;
+  u32 t = blockIdx.x;
+  for (;;) {
+    if (kPersistent) {
+      if (tid == 0) s_tile = atomicAdd(queue, 1u);
+      __syncthreads();
+      t = s_tile;
+    }
+    if (t >= ntiles) break;
+    const TileDesc td = tiles[t];
+    const LayoutDev *L = lays + td.layout;
+    const u32 nb = td.nbits, nw = (nb + 31u) >> 5;
+    const u16 *offrow = L->off + (size_t)td.lane * L->nf;
+    const u32 P = L->P;
+
+    // ---- init: wheel mask, tail mask, and the word-wise certified filters
+    u32 my_pos = 0;
+    {
+      const u32 *mrow = L->mask + (size_t)td.lane * L->mask_words;
+      const u32 kP = P == 1 ? 0u : mod64(td.k0, P, L->Pmagic, L->Pc32);
+      for (u32 w = tid; w < nw; w += bs) {
+        u32 word = 0xFFFFFFFFu;
+        if (P != 1) {
+          const u32 b = mod32(kP + 32u * w, P, L->Pmagic);
+          word = __builtin_amdgcn_alignbit(mrow[(b >> 5) + 1], mrow[b >> 5], b & 31u);
+        }
+        if (w == nw - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
+        my_pos += __popc(word);
+        bits[w] = word;
+      }
+    }
+    apply_small(bits, nw, L, offrow, td.k0, 0, L->cov_small, tid, bs);
+    __syncthreads();
+    // ---- stage B: certified filters
+    apply_scatter(bits, nb, L, offrow, td.k0, L->cov_small, L->ncov, tid, bs);
+    __syncthreads();
+    u32 my_after = 0;
+    if (sieve) {
+      for (u32 w = tid; w < nw; w += bs) my_after += __popc(bits[w]);
+      // ---- stage B': small primes
+      apply_small(bits, nw, L, offrow, td.k0, L->ncov, L->ncov + L->pr_small, tid, bs);
+      __syncthreads();
+      apply_scatter(bits, nb, L, offrow, td.k0, L->ncov + L->pr_small, L->nf, tid, bs);
+      __syncthreads();
+    }
+    u32 my_surv = 0;
+    for (u32 w = tid; w < nw; w += bs) my_surv += __popc(bits[w]);
+    if (!sieve) my_after = my_surv;
+
+    // ---- reductions and the survivor scan (inclusive within the wave)
+    u32 incl = my_surv;
+    for (u32 o = 1; o < 64; o <<= 1) {
+      const u32 n = __shfl_up(incl, o, 64);
+      if (lane64 >= o) incl += n;
+    }
+    const u32 wpos = wave_sum(my_pos), waft = wave_sum(my_after);
+    if (lane64 == 63) s_wsur[wave] = incl;
+    if (lane64 == 0) { s_wpos[wave] = wpos; s_waft[wave] = waft; }
+    __syncthreads();
+    u32 before = 0, tot_s = 0;
+    for (u32 i = 0; i < nwaves; ++i) {
+      if (i < wave) before += s_wsur[i];
+      tot_s += s_wsur[i];
+    }
+    if (tid == 0) {
+      u32 tot_p = 0, tot_a = 0;
+      for (u32 i = 0; i < nwaves; ++i) { tot_p += s_wpos[i]; tot_a += s_waft[i]; }
+      u64 *c = counts + (size_t)td.win * 3;
+      if (tot_p) atomicAdd((unsigned long long *)&c[0], (unsigned long long)tot_p);
+      if (tot_a) atomicAdd((unsigned long long *)&c[1], (unsigned long long)tot_a);
+      if (tot_s) atomicAdd((unsigned long long *)&c[2], (unsigned long long)tot_s);
+      s_base = tot_s ? atomicAdd(surv_count, tot_s) : 0u;
+    }
+    __syncthreads();
+    if (my_surv) {
+      u32 idx = s_base + before + incl - my_surv;
+      const u64 c = L->lanes[td.lane], Mp = L->Mp;
+      for (u32 w = tid; w < nw; w += bs) {
+        u32 x = bits[w];
+        while (x) {
+          const u32 i = __builtin_ctz(x);
+          x &= x - 1;
+          if (idx < cap) {
+            surv_p[idx] = c + Mp * (td.k0 + 32ull * w + i);
+            surv_w[idx] = td.win;
+          }
+          ++idx;
+        }
+      }
+    }
+    if (!kPersistent) break;
+    __syncthreads();   // the next tile reuses the bitmap and s_tile
+  }
+}
+
+__global__ void k_reset(u64 *counts, u32 n, u32 *surv_count, u32 *queue) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counts[i] = 0;
+  if (i == 0) { *surv_count = 0; *queue = 0; }
+}
+
+// ------------------------------------------------------------------ Miller-Rabin
+
+__device__ __forceinline__ u64 mont_mul(u64 a, u64 b, u64 n, u64 ninv) {
+  const u64 lo = a * b, hi = __umul64hi(a, b);
+  const u64 m = lo * ninv;
+  const u64 mh = __umul64hi(m, n);
+  const u64 s = hi + mh;
+  const bool c1 = s < hi;
+  const u64 s2 = s + (lo != 0 ? 1ull : 0ull);
+  const bool c2 = s2 < s;
+  return (c1 || c2 || s2 >= n) ? s2 - n : s2;
+}
+
+__device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 n) {
+  const u64 s = a + b;
+  return (s < a || s >= n) ? s - n : s;
+}
+
+// n odd > 37; d odd with n - 1 = d 2^s; base a small.
+__device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
+  u64 am = 0;                                 // a * 2^64 mod n by double-and-add
+  for (int bit = 5; bit >= 0; --bit) {
+    am = add_mod(am, am, n);
+    if ((a >> bit) & 1) am = add_mod(am, one, n);
+  }
+  const u64 minus_one = n - one;
+  u64 x = one;
+  for (u64 e = d; e; e >>= 1) {
+    if (e & 1) x = mont_mul(x, am, n, ninv);
+    am = mont_mul(am, am, n, ninv);
+  }
+  if (x == one || x == minus_one) return true;
+  for (u32 i = 1; i < s; ++i) {
+    x = mont_mul(x, x, n, ninv);
+    if (x == minus_one) return true;
+  }
+  return false;
+}
+
+__constant__ u64 c_small_p[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+
+__device__ bool is_prime_dev(u64 n) {
+  if (n < 2) return false;
+  if (!(n & 1)) return n == 2;
+  for (int i = 1; i < 12; ++i) {
+    const u64 q = c_small_p[i];
+    if (n == q) return true;
+    if (n % q == 0) return false;
+  }
+  u64 ninv = n;                               // n^{-1} mod 2^64 (Newton), then negate
+  for (int i = 0; i < 6; ++i) ninv *= 2 - n * ninv;
+  ninv = 0 - ninv;
+  const u64 one = (0 - n) % n;                // 2^64 mod n
+  u64 d = n - 1;
+  const u32 s = (u32)__builtin_ctzll(d);
+  d >>= s;
+  if (!strong_probable(n, ninv, one, 2, d, s)) return false;   // the cheap filter first
+  for (int i = 1; i < 12; ++i)
+    if (!strong_probable(n, ninv, one, c_small_p[i], d, s)) return false;
+  return true;
+}
+
+__global__ void k_mr(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
+                     u8 *__restrict__ flags) {
+  const u32 n = min(*count, cap);
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flags[i] = is_prime_dev(vals[i]) ? 1 : 0;
+}
+
+// LDS atomic-AND throughput probe: every thread clears bits of its own word for `iters`
+// rounds (consecutive threads on consecutive banks, so no bank conflicts); the word is
+// written out at the end so the work is observable.
+__global__ __launch_bounds__(256) void k_lds_peak(u32 iters, u32 *sink) {
+  __shared__ u32 w[1024];
+  const u32 tid = threadIdx.x;
+  for (u32 i = tid; i < 1024; i += blockDim.x) w[i] = 0xFFFFFFFFu;
+  __syncthreads();
+  u32 slot = tid;
+  for (u32 i = 0; i < iters; ++i) {
+    atomicAnd(&w[slot], ~(1u << (i & 31u)));
+    slot = (slot + 256u) & 1023u;
+  }
+  __syncthreads();
+  if (w[tid] == 0x12345678u) sink[blockIdx.x] = tid;   // never true; keeps the loop live
+}
+
+}  // namespace dev
+
+// ====================================================================== host side
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      err_ = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+      return -1;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+struct DevLayoutBufs {
+  dev::LayoutDev d;
+  void *lanes = nullptr, *mask = nullptr, *step = nullptr, *magic = nullptr, *c32 = nullptr,
+       *off = nullptr;
+};
+
+struct DeviceCtx::Impl {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  std::vector<DevLayoutBufs> layouts;
+  dev::LayoutDev *d_lays = nullptr;
+  size_t lays_cap = 0;
+  TileDesc *d_tiles = nullptr;
+  size_t tiles_cap = 0;
+  dev::BatchHdr *d_hdr = nullptr;
+  dev::BatchHdr *h_hdr = nullptr;
+  u64 *d_counts = nullptr;
+  size_t win_cap = 0;
+  u64 *d_surv_p = nullptr;
+  u32 *d_surv_w = nullptr;
+  uint8_t *d_flags = nullptr;
+  size_t surv_cap = 0;
+  u32 *d_scalars = nullptr;   // [0] survivor count, [1] queue head
+  u32 *h_count = nullptr;
+  int occupancy_blocks = 0;
+  u32 occ_block = 0, occ_bits = 0;
+};
+
+DeviceCtx::DeviceCtx() : impl_(new Impl) {}
+
+DeviceCtx::~DeviceCtx() {
+  if (!impl_) return;
+  Impl &m = *impl_;
+  if (device_ >= 0) {
+    (void)hipSetDevice(device_);
+    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    for (DevLayoutBufs &b : m.layouts)
+      for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.off}) if (p) (void)hipFree(p);
+    for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_counts,
+                    (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_scalars})
+      if (p) (void)hipFree(p);
+    if (m.h_hdr) (void)hipHostFree(m.h_hdr);
+    if (m.h_count) (void)hipHostFree(m.h_count);
+    for (hipEvent_t e : {m.ev0, m.ev1, m.ev2}) if (e) (void)hipEventDestroy(e);
+    if (m.stream) (void)hipStreamDestroy(m.stream);
+  }
+  delete impl_;
+}
+
+std::string DeviceCtx::device_name() const {
+  hipDeviceProp_t p;
+  if (device_ < 0 || hipGetDeviceProperties(&p, device_) != hipSuccess) return "?";
+  return std::string(p.name) + " (" + p.gcnArchName + ")";
+}
+
+int DeviceCtx::open(int device) {
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) { err_ = "no such HIP device"; return -1; }
+  HIP_TRY(hipSetDevice(device));
+  device_ = device;
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, device));
+  cus_ = p.multiProcessorCount;
+  Impl &m = *impl_;
+  HIP_TRY(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&m.ev0));
+  HIP_TRY(hipEventCreate(&m.ev1));
+  HIP_TRY(hipEventCreate(&m.ev2));
+  HIP_TRY(hipMalloc(&m.d_hdr, sizeof(dev::BatchHdr)));
+  HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
+  HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
+  HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32)));
+  return 0;
+}
+
+int DeviceCtx::add_layout(const Layout &L) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  DevLayoutBufs b;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    HIP_TRY(hipMalloc(dst, bytes ? bytes : 8));
+    if (bytes) HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    layout_bytes_ += bytes;
+    return 0;
+  };
+  if (up(&b.lanes, L.lanes.data(), L.lanes.size() * 8) || up(&b.mask, L.mask.data(), L.mask.size() * 4) ||
+      up(&b.step, L.step.data(), L.step.size() * 4) || up(&b.magic, L.magic.data(), L.magic.size() * 4) ||
+      up(&b.c32, L.c32.data(), L.c32.size() * 4) || up(&b.off, L.off.data(), L.off.size() * 2))
+    return -1;
+  dev::LayoutDev &d = b.d;
+  d.Mp = L.Mp;
+  d.lanes = (const u64 *)b.lanes;
+  d.mask = (const u32 *)b.mask;
+  d.step = (const u32 *)b.step;
+  d.magic = (const u32 *)b.magic;
+  d.c32 = (const u32 *)b.c32;
+  d.off = (const u16 *)b.off;
+  d.P = (u32)L.P;
+  d.Pmagic = L.P == 1 ? 0xFFFFFFFFu : (u32)((1ull << 32) / L.P);
+  d.Pc32 = (u32)((1ull << 32) % L.P);
+  d.nl = (u32)L.lanes.size();
+  d.nf = L.nf;
+  d.ncov = L.ncov;
+  d.cov_small = L.cov_small;
+  d.pr_small = L.pr_small;
+  d.mask_words = L.mask_words;
+  d.pad = 0;
+  m.layouts.push_back(b);
+  // Mirror the descriptor array on the device.
+  if (m.layouts.size() > m.lays_cap) {
+    if (m.d_lays) HIP_TRY(hipFree(m.d_lays));
+    m.lays_cap = std::max<size_t>(8, m.layouts.size() * 2);
+    HIP_TRY(hipMalloc(&m.d_lays, m.lays_cap * sizeof(dev::LayoutDev)));
+  }
+  std::vector<dev::LayoutDev> all;
+  for (const DevLayoutBufs &x : m.layouts) all.push_back(x.d);
+  HIP_TRY(hipMemcpy(m.d_lays, all.data(), all.size() * sizeof(dev::LayoutDev), hipMemcpyHostToDevice));
+  return (int)m.layouts.size() - 1;
+}
+
+int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
+                   BatchResult &out) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  const size_t nt = tiles.size();
+  out.counts.assign(nwin, WinCounts{0, 0, 0});
+  out.surv_p.clear();
+  out.surv_w.clear();
+  out.surv_prime.clear();
+  out.replays = 0;
+  out.device_ms = out.sieve_ms = 0;
+  if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32) {
+    err_ = "bad launch configuration";
+    return -1;
+  }
+  // Host-side validation of every descriptor against the tables it indexes.
+  for (const TileDesc &t : tiles) {
+    if (t.layout >= m.layouts.size() || t.lane >= m.layouts[t.layout].d.nl || t.win >= nwin ||
+        t.nbits == 0 || t.nbits > cfg.max_bits) {
+      err_ = "tile descriptor out of range";
+      return -1;
+    }
+  }
+  if (nt == 0) return 0;
+  // Capacities.
+  if (nt > m.tiles_cap) {
+    if (m.d_tiles) HIP_TRY(hipFree(m.d_tiles));
+    m.tiles_cap = std::max<size_t>(nt, 2 * m.tiles_cap);
+    HIP_TRY(hipMalloc(&m.d_tiles, m.tiles_cap * sizeof(TileDesc)));
+  }
+  if (nwin > m.win_cap) {
+    if (m.d_counts) HIP_TRY(hipFree(m.d_counts));
+    m.win_cap = std::max<size_t>(nwin, 2 * m.win_cap);
+    HIP_TRY(hipMalloc(&m.d_counts, m.win_cap * 3 * sizeof(u64)));
+  }
+  auto grow_surv = [&](size_t need) -> int {
+    if (need <= m.surv_cap) return 0;
+    size_t cap = std::max<size_t>(m.surv_cap ? m.surv_cap : (1u << 16), 1u << 16);
+    while (cap < need) cap *= 2;
+    if (cap > 0xFFFFFFFFull) { err_ = "survivor count exceeds 2^32"; return -1; }
+    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags}) if (p) HIP_TRY(hipFree(p));
+    HIP_TRY(hipMalloc(&m.d_surv_p, cap * sizeof(u64)));
+    HIP_TRY(hipMalloc(&m.d_surv_w, cap * sizeof(u32)));
+    HIP_TRY(hipMalloc(&m.d_flags, cap));
+    m.surv_cap = cap;
+    return 0;
+  };
+  if (grow_surv(1u << 16)) return -1;
+
+  m.h_hdr->ntiles = (u32)nt;
+  m.h_hdr->nwin = nwin;
+  m.h_hdr->has_sieve = has_sieve ? 1 : 0;
+  m.h_hdr->pad = 0;
+  HIP_TRY(hipMemcpyAsync(m.d_hdr, m.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemcpyAsync(m.d_tiles, tiles.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
+
+  const size_t lds = cfg.max_bits / 8;
+  if (m.occ_block != cfg.block || m.occ_bits != cfg.max_bits) {
+    int nb = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, lds));
+    m.occupancy_blocks = std::max(1, nb);
+    m.occ_block = cfg.block;
+    m.occ_bits = cfg.max_bits;
+  }
+  for (;;) {
+    const u32 ncnt = nwin * 3;
+    dev::k_reset<<<(ncnt + 255) / 256 ? (ncnt + 255) / 256 : 1, 256, 0, m.stream>>>(
+        m.d_counts, ncnt, m.d_scalars, m.d_scalars + 1);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m.ev0, m.stream));
+    if (cfg.persistent) {
+      const u32 grid = (u32)std::min<size_t>(nt, (size_t)cus_ * m.occupancy_blocks);
+      dev::k_sieve<true><<<grid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
+                                                             m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
+                                                             m.d_scalars, m.d_scalars + 1);
+    } else {
+      dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
+                                                                 m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
+                                                                 m.d_scalars, m.d_scalars + 1);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m.ev1, m.stream));
+    dev::k_mr<<<std::max(1, cus_ * 2), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+                                                            m.d_flags);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m.ev2, m.stream));
+    HIP_TRY(hipMemcpyAsync(m.h_count, m.d_scalars, sizeof(u32), hipMemcpyDeviceToHost, m.stream));
+    HIP_TRY(hipStreamSynchronize(m.stream));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, m.ev0, m.ev1));
+    HIP_TRY(hipEventElapsedTime(&b, m.ev0, m.ev2));
+    out.sieve_ms += a;
+    out.device_ms += b;
+    const u32 cnt = *m.h_count;
+    if (cnt <= m.surv_cap) break;
+    // Overflow: the counts are complete, the list is not. Grow and replay the batch
+    // (sieve/cuda/scan.cu:223-242 does the same).
+    ++out.replays;
+    if (grow_surv(cnt)) return -1;
+  }
+  const u32 cnt = *m.h_count;
+  std::vector<u64> raw(nwin * 3);
+  HIP_TRY(hipMemcpyAsync(raw.data(), m.d_counts, nwin * 3 * sizeof(u64), hipMemcpyDeviceToHost, m.stream));
+  out.surv_p.resize(cnt);
+  out.surv_w.resize(cnt);
+  out.surv_prime.resize(cnt);
+  if (cnt) {
+    HIP_TRY(hipMemcpyAsync(out.surv_p.data(), m.d_surv_p, cnt * sizeof(u64), hipMemcpyDeviceToHost, m.stream));
+    HIP_TRY(hipMemcpyAsync(out.surv_w.data(), m.d_surv_w, cnt * sizeof(u32), hipMemcpyDeviceToHost, m.stream));
+    HIP_TRY(hipMemcpyAsync(out.surv_prime.data(), m.d_flags, cnt, hipMemcpyDeviceToHost, m.stream));
+  }
+  HIP_TRY(hipStreamSynchronize(m.stream));
+  for (u32 w = 0; w < nwin; ++w) out.counts[w] = WinCounts{raw[w * 3], raw[w * 3 + 1], raw[w * 3 + 2]};
+  return 0;
+}
+
+int DeviceCtx::lds_peak(double &ops_per_s) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  u32 *sink = nullptr;
+  const u32 blocks = (u32)cus_ * 8, iters = 1u << 14;
+  HIP_TRY(hipMalloc(&sink, blocks * sizeof(u32)));
+  dev::k_lds_peak<<<blocks, 256, 0, m.stream>>>(64, sink);   // warm-up
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m.ev0, m.stream));
+  dev::k_lds_peak<<<blocks, 256, 0, m.stream>>>(iters, sink);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m.ev1, m.stream));
+  HIP_TRY(hipStreamSynchronize(m.stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, m.ev0, m.ev1));
+  HIP_TRY(hipFree(sink));
+  ops_per_s = (double)blocks * 256.0 * iters / (ms * 1e-3);
+  return 0;
+}
+
+int DeviceCtx::primality(const u64 *vals, size_t n, uint8_t *flags) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull) { err_ = "too many values"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  u64 *d_v = nullptr;
+  uint8_t *d_f = nullptr;
+  u32 *d_n = nullptr;
+  HIP_TRY(hipMalloc(&d_v, n * 8));
+  HIP_TRY(hipMalloc(&d_f, n));
+  HIP_TRY(hipMalloc(&d_n, 4));
+  const u32 n32 = (u32)n;
+  HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
+  dev::k_mr<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(d_v, d_n, n32, d_f);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(flags, d_f, n, hipMemcpyDeviceToHost, m.stream));
+  HIP_TRY(hipStreamSynchronize(m.stream));
+  HIP_TRY(hipFree(d_v));
+  HIP_TRY(hipFree(d_f));
+  HIP_TRY(hipFree(d_n));
+  return 0;
+}
+
+}  // namespace es

@@ -1,0 +1,333 @@
+// es_scan.cpp -- windows -> tiles -> device batch -> host tail -> records.
+#include "es_scan.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <thread>
+
+#include "es_tail.h"
+
+namespace es {
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+u64 gcd64(u64 a, u64 b) {
+  while (b) { const u64 t = a % b; a = b; b = t; }
+  return a;
+}
+
+}  // namespace
+
+void parallel_for(size_t n, int threads, const std::function<void(size_t)> &fn) {
+  if (n == 0) return;
+  int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  T = (int)std::min<size_t>((size_t)T, n);
+  if (T <= 1) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  pool.reserve(T);
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&]() {
+      for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+    });
+  for (std::thread &th : pool) th.join();
+}
+
+void shard_interval(u64 &lo, u64 &hi, u64 shard, u64 nshard) {
+  if (nshard == 0) nshard = 1;
+  if (nshard > 1) {
+    const u64 span = (hi - lo + nshard - 1) / nshard;
+    const u64 a = lo + shard * span, b = std::min(hi, a + span);
+    lo = a;
+    hi = std::max(a, b);
+  }
+}
+
+int Scanner::init_wheel(u64 M, u64 mmax, std::string &msg) {
+  return build_plan(M, mmax, plan_, msg);
+}
+
+int Scanner::init_filters(const char *filter_file, u64 sieve, std::string &msg) {
+  if (plan_filters(plan_, filter_file, sieve, msg)) return 1;
+  // Per candidate sub-wheel: lanes, and expected marks per integer from how many lanes
+  // each filter hits (lanes c' with c' == res mod gcd(m, Mp)).
+  est_.clear();
+  const FilterSet &fs = plan_.filters;
+  for (u64 Mp : layout_candidates(plan_.M)) {
+    std::vector<u64> lanes;
+    for (u64 c : plan_.wheel.classes) lanes.push_back(c % Mp);
+    std::sort(lanes.begin(), lanes.end());
+    lanes.erase(std::unique(lanes.begin(), lanes.end()), lanes.end());
+    std::map<u64, std::map<u64, u64>> hist;   // g -> residue -> lanes
+    double marks = 0;
+    for (size_t i = 0; i < fs.mod.size(); ++i) {
+      const u64 m = fs.mod[i], g = gcd64(m, Mp), s = m / g;
+      auto it = hist.find(g);
+      if (it == hist.end()) {
+        std::map<u64, u64> h;
+        for (u64 c : lanes) ++h[c % g];
+        it = hist.emplace(g, std::move(h)).first;
+      }
+      auto jt = it->second.find(fs.res[i] % m % g);
+      if (jt != it->second.end()) marks += (double)jt->second / (double)s;
+    }
+    est_.push_back(Estimate{Mp, (double)lanes.size(), marks / (double)Mp, (double)lanes.size() / (double)Mp});
+  }
+  return 0;
+}
+
+int Scanner::open_device(int device, std::string &msg) {
+  if (dev_.open(device)) { msg = "FATAL: HIP device: " + dev_.last_error(); return -1; }
+  dev_open_ = true;
+  return 0;
+}
+
+u64 Scanner::choose_layout(u64 w) const {
+  if (tune_.fixed_Mp) return tune_.fixed_Mp;
+  const double nf = (double)plan_.filters.mod.size();
+  const double maxb = (double)tune_.launch.max_bits;
+  u64 best = plan_.M;
+  double best_cost = 1e300;
+  for (const Estimate &e : est_) {
+    const double per_lane = (double)w / (double)e.Mp;
+    const double tiles = e.nl * (per_lane <= 1.0 ? per_lane : std::ceil(per_lane / maxb));
+    const double cost = tiles * nf * tune_.c_pair +
+                        (double)w * (e.marks_per_int * tune_.c_mark + e.bits_per_int * tune_.c_bit);
+    if (cost < best_cost) { best_cost = cost; best = e.Mp; }
+  }
+  return best;
+}
+
+int Scanner::mark_count(u64 lo, u64 hi, u64 Mp, u64 &marks, u64 &bits, u64 &pairs, u64 &tiles,
+                        std::string &msg) {
+  marks = bits = pairs = tiles = 0;
+  if (hi <= lo) return 0;
+  if (!Mp) Mp = choose_layout(hi - lo);
+  const Layout *L = host_layout(Mp, msg);
+  if (!L) return -1;
+  const u64 maxb = tune_.launch.max_bits;
+  for (size_t i = 0; i < L->lanes.size(); ++i) {
+    u64 k0, nb;
+    lane_span(lo, hi, L->lanes[i], Mp, k0, nb);
+    for (u64 b = 0; b < nb; b += maxb) {
+      const u64 kt = k0 + b, n = std::min(maxb, nb - b);
+      ++tiles;
+      bits += n;
+      const u16 *row = L->off.data() + i * L->nf;
+      for (u32 j = 0; j < L->nf; ++j) {
+        if (row[j] == kNoHit) continue;
+        const u64 s = L->step[j], o = row[j], km = kt % s;
+        const u64 first = o >= km ? o - km : o + s - km;
+        ++pairs;
+        if (first < n) marks += (n - 1 - first) / s + 1;
+      }
+    }
+  }
+  return 0;
+}
+
+const Layout *Scanner::layout(u64 Mp) const {
+  auto it = layouts_.find(Mp);
+  return it == layouts_.end() ? nullptr : it->second.get();
+}
+
+const Layout *Scanner::host_layout(u64 Mp, std::string &msg) {
+  auto it = layouts_.find(Mp);
+  if (it != layouts_.end()) return it->second.get();
+  std::unique_ptr<Layout> L(new Layout);
+  if (!build_layout(plan_.M, Mp, plan_.wheel.classes, plan_.filters, *L, msg)) {
+    msg = "FATAL: layout " + std::to_string(Mp) + ": " + msg;
+    return nullptr;
+  }
+  const Layout *p = L.get();
+  layouts_[Mp] = std::move(L);
+  return p;
+}
+
+int Scanner::ensure_layout(u64 Mp, std::string &msg) {
+  auto it = layout_id_.find(Mp);
+  if (it != layout_id_.end()) return it->second;
+  if (!dev_open_) { msg = "FATAL: HIP device not open"; return -1; }
+  const Layout *L = host_layout(Mp, msg);
+  if (!L) return -1;
+  const int id = dev_.add_layout(*L);
+  if (id < 0) { msg = "FATAL: HIP device: " + dev_.last_error(); return -1; }
+  layout_id_[Mp] = id;
+  return id;
+}
+
+int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                         ScanTiming &tm, std::string &msg) {
+  const double t0 = now_ms();
+  tm = ScanTiming();
+  out.assign(wins.size(), WindowRecord());
+  const u32 maxb = tune_.launch.max_bits;
+  std::vector<TileDesc> tiles;
+  for (size_t w = 0; w < wins.size(); ++w) {
+    WindowRecord &r = out[w];
+    r.lo = wins[w].lo;
+    r.hi = wins[w].hi;
+    if (r.hi <= r.lo) continue;
+    const u64 Mp = choose_layout(r.hi - r.lo);
+    const int id = ensure_layout(Mp, msg);
+    if (id < 0) return -1;
+    r.layout_Mp = Mp;
+    const Layout &L = *layouts_[Mp];
+    for (u32 i = 0; i < (u32)L.lanes.size(); ++i) {
+      u64 k0, nbits;
+      lane_span(r.lo, r.hi, L.lanes[i], Mp, k0, nbits);
+      for (u64 b = 0; b < nbits; b += maxb)
+        tiles.push_back(TileDesc{k0 + b, (u32)std::min<u64>(maxb, nbits - b), i, (u32)w, (u32)id});
+    }
+  }
+  // Longest tiles first: the persistent queue then ends with the short ones.
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const TileDesc &a, const TileDesc &b) { return a.nbits > b.nbits; });
+  tm.tiles = tiles.size();
+  const double t1 = now_ms();
+  tm.plan_ms = t1 - t0;
+
+  if (!dev_open_) { msg = "FATAL: HIP device not open"; return -1; }
+  if (dev_.run(tiles, (u32)wins.size(), plan_.filters.mod.size() != plan_.filters.ncov, tune_.launch, br)) {
+    msg = "FATAL: HIP device: " + dev_.last_error();
+    return -1;
+  }
+  const double t2 = now_ms();
+  tm.batch_ms = t2 - t1;
+  tm.device_ms = br.device_ms;
+  tm.sieve_ms = br.sieve_ms;
+  tm.replays = br.replays;
+  tm.survivors = br.surv_p.size();
+
+  // Counters (rung_scan3.cpp:764-772): covered counts cover kills only.
+  const bool sieve = plan_.filters.mod.size() != plan_.filters.ncov;
+  for (size_t w = 0; w < wins.size(); ++w) {
+    WindowRecord &r = out[w];
+    const WinCounts &c = br.counts[w];
+    r.positions = c.positions;
+    r.mr = c.survivors;
+    if (!sieve) {
+      r.covered = c.positions - c.survivors;
+    } else {
+      r.covered = c.positions - c.after_covers;
+      r.sieved = c.after_covers - c.survivors;
+    }
+  }
+  for (size_t i = 0; i < br.surv_p.size(); ++i)
+    if (!br.surv_prime[i]) ++out[br.surv_w[i]].composite;
+  tm.total_ms = now_ms() - t0;
+  return 0;
+}
+
+int Scanner::survivors(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, std::vector<u64> &p,
+                       std::vector<uint8_t> &prime, ScanTiming &tm, std::string &msg) {
+  BatchResult br;
+  if (device_pass(wins, out, br, tm, msg)) return -1;
+  std::vector<size_t> idx(br.surv_p.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    return br.surv_w[a] != br.surv_w[b] ? br.surv_w[a] < br.surv_w[b] : br.surv_p[a] < br.surv_p[b];
+  });
+  p.resize(idx.size());
+  prime.resize(idx.size());
+  for (size_t i = 0; i < idx.size(); ++i) {
+    p[i] = br.surv_p[idx[i]];
+    prime[i] = br.surv_prime[idx[i]];
+  }
+  return 0;
+}
+
+int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, ScanTiming &tm,
+                  std::string &msg) {
+  BatchResult br;
+  if (device_pass(wins, out, br, tm, msg)) return -1;
+  const double t2 = now_ms();
+  const double t0 = t2 - tm.total_ms;
+  const u64 M = plan_.M;
+  std::vector<size_t> prime_idx;
+  for (size_t i = 0; i < br.surv_p.size(); ++i)
+    if (br.surv_prime[i]) prime_idx.push_back(i);
+  tm.primes = prime_idx.size();
+
+  // Host tail: stage D then E per prime, in parallel; lines land in per-prime slots.
+  std::vector<std::string> lines(prime_idx.size());
+  std::vector<int> kinds(prime_idx.size());
+  parallel_for(prime_idx.size(), threads, [&](size_t j) {
+    kinds[j] = certify_prime(br.surv_p[prime_idx[j]], lines[j]);
+  });
+  const double t3 = now_ms();
+  tm.tail_ms = t3 - t2;
+
+  // Emission order.
+  const std::vector<u64> &cls = plan_.wheel.classes;
+  struct Keyed { u64 k1, k2, p; size_t j; };
+  std::vector<std::vector<Keyed>> per(wins.size());
+  for (size_t j = 0; j < prime_idx.size(); ++j) {
+    const size_t i = prime_idx[j];
+    const u64 p = br.surv_p[i];
+    const u32 w = br.surv_w[i];
+    const u64 rho = p % M;
+    const u64 rank = (u64)(std::lower_bound(cls.begin(), cls.end(), rho) - cls.begin());
+    u64 k1 = rank, k2 = 0;
+    if (order == Order::Round) {
+      const u64 lo = wins[w].lo;
+      const u64 first = lo + (rho + M - lo % M) % M;
+      k1 = ((p - first) / M) / round_seg;
+      k2 = rank;
+    }
+    per[w].push_back(Keyed{k1, k2, p, j});
+  }
+  for (size_t w = 0; w < wins.size(); ++w) {
+    std::vector<Keyed> &v = per[w];
+    std::sort(v.begin(), v.end(), [](const Keyed &a, const Keyed &b) {
+      if (a.k1 != b.k1) return a.k1 < b.k1;
+      if (a.k2 != b.k2) return a.k2 < b.k2;
+      return a.p < b.p;
+    });
+    WindowRecord &r = out[w];
+    for (const Keyed &k : v) {
+      const int kind = kinds[k.j];
+      if (kind == 0) ++r.rung;
+      else if (kind == 1) ++r.direct;
+      else ++r.survivors;
+      r.lines.push_back(std::move(lines[k.j]));
+      r.line_p.push_back(k.p);
+    }
+  }
+  tm.total_ms = now_ms() - t0;
+  return 0;
+}
+
+std::string Scanner::record_text(const WindowRecord &r, bool emit_survivors) const {
+  std::string s;
+  for (const std::string &l : r.lines)
+    if (emit_survivors || l.compare(0, 9, "SURVIVOR ") != 0) s += l;
+  char b[640];
+  snprintf(b, sizeof b,
+           "SUMMARY lo=%" PRIu64 " hi=%" PRIu64 " wheelM=%" PRIu64 " lanes=%zu mmax=%" PRIu64
+           " ucert_rmax=%u ucert_umax=%u wheel_pmin=%" PRIu64 " classes_sha256=%s positions=%" PRIu64
+           " covered=%" PRIu64 " sieved=%" PRIu64 " mr=%" PRIu64 " composite=%" PRIu64 " rung=%" PRIu64
+           " direct=%" PRIu64 " survivors=%" PRIu64 "\n",
+           r.lo, r.hi, plan_.M, plan_.wheel.classes.size(), plan_.mmax, kUcertRmax, kUcertUmax,
+           plan_.wheel.max_pmin, plan_.pin ? plan_.pin->sha256 : "unpinned", r.positions, r.covered, r.sieved,
+           r.mr, r.composite, r.rung, r.direct, r.survivors);
+  s += b;
+  return s;
+}
+
+}  // namespace es

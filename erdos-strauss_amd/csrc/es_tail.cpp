@@ -1,0 +1,341 @@
+// es_tail.cpp -- stage D/E on the host. Montgomery multiplication modulo odd n < 2^64
+// (REDC with the explicit high-half carry, so n >= 2^63 is exact), magic-inverse trial
+// division, Brent's rho, and the per-residue minimum over divisors of q^2.
+#include "es_tail.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+namespace es {
+
+namespace {
+
+constexpr u64 kTrialBound = 65536;   // census/rung_scan.cpp:175
+constexpr u64 kDirectNMax = 18446744065119617025ull;
+constexpr unsigned kRungCap = 255;
+
+u64 gcd64(u64 a, u64 b) {
+  while (b) { const u64 t = a % b; a = b; b = t; }
+  return a;
+}
+
+// Montgomery form modulo an odd n with R = 2^64.
+struct Mont {
+  u64 n, nneg_inv, r1, r2;
+  static u64 inv2_64(u64 a) {            // a^{-1} mod 2^64 for odd a (Newton, 6 steps)
+    u64 x = a;
+    for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+    return x;
+  }
+  explicit Mont(u64 n_) : n(n_) {
+    nneg_inv = 0 - inv2_64(n_);
+    r1 = (u64)(((u128)1 << 64) % n_);
+    r2 = (u64)((u128)r1 * r1 % n_);
+  }
+  // t < n * 2^64: returns t / 2^64 mod n. The low half of t + m n is zero, so the sum
+  // of the high halves plus the carry out of the low half is the result; it can reach
+  // 2n - 1 >= 2^64 when n >= 2^63, hence the u128 accumulation.
+  u64 redc(u128 t) const {
+    const u64 m = (u64)t * nneg_inv;
+    const u128 s = (t >> 64) + (((u128)m * n) >> 64) + ((u64)t != 0);
+    return (u64)(s >= n ? s - n : s);
+  }
+  u64 mul(u64 a, u64 b) const { return redc((u128)a * b); }
+  u64 to(u64 a) const { return mul(a % n, r2); }
+  u64 add(u64 a, u64 b) const {
+    const u64 s = a + b;
+    return (s < a || s >= n) ? s - n : s;
+  }
+};
+
+const u64 kBases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+
+// n odd > 37; d odd with n-1 = d 2^s; a in Montgomery form.
+bool strong_probable(const Mont &M, u64 a_m, u64 d, int s) {
+  const u64 one = M.r1, minus_one = M.n - M.r1;
+  u64 x = one, b = a_m;
+  for (u64 e = d; e; e >>= 1) {
+    if (e & 1) x = M.mul(x, b);
+    b = M.mul(b, b);
+  }
+  if (x == one || x == minus_one) return true;
+  for (int i = 1; i < s; ++i) {
+    x = M.mul(x, x);
+    if (x == minus_one) return true;
+  }
+  return false;
+}
+
+// Trial-division tables: odd primes 3..65536 with q^{-1} mod 2^64 and floor((2^64-1)/q):
+// q | n  <=>  n * q^{-1} mod 2^64 <= floor((2^64-1)/q).
+struct TrialTables {
+  std::vector<u64> q, inv, lim, q2;
+  TrialTables() {
+    std::vector<uint8_t> comp(kTrialBound + 1, 0);
+    for (u64 i = 3; i <= kTrialBound; i += 2) {
+      if (comp[i]) continue;
+      q.push_back(i);
+      inv.push_back(Mont::inv2_64(i));
+      lim.push_back(~0ull / i);
+      q2.push_back(i * i);
+      for (u64 j = i * i; j <= kTrialBound; j += 2 * i) comp[j] = 1;
+    }
+  }
+};
+const TrialTables &trial() {
+  static const TrialTables t;
+  return t;
+}
+
+struct Fac {
+  u64 p[16];
+  int e[16];
+  int k = 0;
+  bool overflow = false;
+  void add(u64 q, int x) {
+    for (int i = 0; i < k; ++i)
+      if (p[i] == q) { e[i] += x; return; }
+    if (k >= 16) { overflow = true; return; }
+    p[k] = q; e[k] = x; ++k;
+  }
+};
+
+u64 isqrt64(u64 n) {
+  u64 s = (u64)std::sqrt((long double)n);
+  if (s > 0xFFFFFFFFull) s = 0xFFFFFFFFull;
+  while (s && s * s > n) --s;
+  while (s < 0xFFFFFFFFull && (s + 1) * (s + 1) <= n) ++s;
+  return s;
+}
+
+// Brent's rho in Montgomery form; returns a nontrivial factor of odd composite n.
+u64 rho_factor(u64 n) {
+  const Mont M(n);
+  for (u64 c = 1;; ++c) {
+    const u64 cm = M.to(c);
+    auto f = [&](u64 v) { return M.add(M.mul(v, v), cm); };
+    u64 y = M.to(2), x = y, ys = y, acc = M.r1, g = 1;
+    const u64 m = 128;
+    for (u64 r = 1; g == 1 && r <= (1ull << 26); r <<= 1) {
+      x = y;
+      for (u64 i = 0; i < r; ++i) y = f(y);
+      for (u64 k = 0; k < r && g == 1; k += m) {
+        ys = y;
+        const u64 lim = std::min(m, r - k);
+        for (u64 i = 0; i < lim; ++i) {
+          y = f(y);
+          acc = M.mul(acc, x > y ? x - y : y - x);
+        }
+        g = gcd64(acc, n);
+      }
+    }
+    if (g == n || g == 1) {               // the batch overshot: replay it one step at a time
+      g = 1;
+      for (u64 guard = 0; g == 1 && guard < 256; ++guard) {
+        ys = f(ys);
+        const u64 diff = x > ys ? x - ys : ys - x;
+        g = gcd64(diff ? diff : n, n);
+      }
+    }
+    if (g != 1 && g != n) return g;
+  }
+}
+
+void split(u64 n, Fac &f) {
+  if (n == 1) return;
+  if (is_prime_u64(n)) { f.add(n, 1); return; }
+  const u64 s = isqrt64(n);
+  if (s * s == n) {
+    Fac g;
+    split(s, g);
+    for (int i = 0; i < g.k; ++i) f.add(g.p[i], 2 * g.e[i]);
+    f.overflow |= g.overflow;
+    return;
+  }
+  const u64 d = rho_factor(n);
+  split(d, f);
+  split(n / d, f);
+}
+
+void factor(u64 n, Fac &f) {
+  f.k = 0;
+  f.overflow = false;
+  if (n == 0) return;
+  if (!(n & 1)) {
+    const int e = __builtin_ctzll(n);
+    n >>= e;
+    f.add(2, e);
+  }
+  const TrialTables &t = trial();
+  for (size_t i = 0; i < t.q.size(); ++i) {
+    if (t.q2[i] > n) break;
+    if (n * t.inv[i] > t.lim[i]) continue;
+    int e = 0;
+    do { n /= t.q[i]; ++e; } while (n * t.inv[i] <= t.lim[i]);
+    f.add(t.q[i], e);
+  }
+  if (n > 1) {
+    if (n <= kTrialBound * kTrialBound) f.add(n, 1);   // all prime factors exceed 65536
+    else split(n, f);
+  }
+}
+
+// Least u | q^2 with u == target (mod r) and u <= q (0 if none). best[s] is the least
+// divisor built so far in residue class s; multiplying by a prime power preserves the
+// order within a class, so the per-class minimum is all that needs carrying, and
+// nothing above q needs to exist (rung_scan3.cpp:534-565).
+u128 least_witness(u64 p, const Fac &fa, u128 q, unsigned r, unsigned target) {
+  const u128 NONE = ~(u128)0;
+  u128 best[kRungCap + 1], next[kRungCap + 1];
+  for (unsigned s = 0; s < r; ++s) best[s] = NONE;
+  best[1 % r] = 1;
+  auto absorb = [&](u64 pr, int e) {
+    for (unsigned s = 0; s < r; ++s) next[s] = NONE;
+    const unsigned pm = (unsigned)(pr % r);
+    const u128 cap = q / pr;
+    for (unsigned s = 0; s < r; ++s) {
+      if (best[s] == NONE) continue;
+      u128 v = best[s];
+      unsigned res = s;
+      for (int j = 0;; ++j) {
+        if (v < next[res]) next[res] = v;
+        if (j == e || v > cap) break;
+        v *= pr;
+        res = (res * pm) % r;
+      }
+    }
+    for (unsigned s = 0; s < r; ++s) best[s] = next[s];
+  };
+  absorb(p, 2);
+  for (int i = 0; i < fa.k; ++i) absorb(fa.p[i], 2 * fa.e[i]);
+  return best[target] == NONE ? 0 : best[target];
+}
+
+void append_u128(std::string &s, u128 v) {
+  char buf[48];
+  int n = 0;
+  do { buf[n++] = (char)('0' + (int)(v % 10)); v /= 10; } while (v);
+  while (n--) s.push_back(buf[n]);
+}
+
+}  // namespace
+
+bool is_prime_u64(u64 n) {
+  if (n < 2) return false;
+  for (u64 q : kBases) {
+    if (n == q) return true;
+    if (n % q == 0) return false;
+  }
+  const Mont M(n);
+  u64 d = n - 1;
+  const int s = __builtin_ctzll(d);
+  d >>= s;
+  for (u64 a : kBases)
+    if (!strong_probable(M, M.to(a), d, s)) return false;
+  return true;
+}
+
+bool sprp2_u64(u64 n) {
+  if (n < 2) return false;
+  if (!(n & 1)) return n == 2;
+  if (n == 3) return true;
+  const Mont M(n);
+  u64 d = n - 1;
+  const int s = __builtin_ctzll(d);
+  d >>= s;
+  return strong_probable(M, M.to(2), d, s);
+}
+
+int factor_u64(u64 n, u64 *primes, int *exps) {
+  Fac f;
+  factor(n, f);
+  if (f.overflow) return -1;
+  for (int i = 1; i < f.k; ++i)
+    for (int j = i; j > 0 && f.p[j - 1] > f.p[j]; --j) {
+      std::swap(f.p[j - 1], f.p[j]);
+      std::swap(f.e[j - 1], f.e[j]);
+    }
+  for (int i = 0; i < f.k; ++i) { primes[i] = f.p[i]; exps[i] = f.e[i]; }
+  return f.k;
+}
+
+// rung_scan3.cpp:574-603: rungs r == -p (mod 4) from the smallest, skipping those with
+// gcd(pA mod r, r) > 1; the first with a witness wins.
+bool solve_rung(u64 p, std::string &line) {
+  unsigned r = (unsigned)((4 - p % 4) % 4);
+  if (r == 0) r = 4;
+  for (; r <= kRungCap; r += 4) {
+    const u64 A = (p + r) / 4;
+    if (!A) continue;
+    const u128 q = (u128)p * A;
+    const unsigned qm = (unsigned)(q % r);
+    if (r > 1 && gcd64(qm, r) != 1) continue;
+    const unsigned target = qm ? r - qm : 0;
+    Fac fa;
+    factor(A, fa);
+    if (fa.overflow) {
+      fprintf(stderr, "FATAL: more than 16 distinct prime factors\n");
+      abort();
+    }
+    const u128 u = least_witness(p, fa, q, r, target);
+    if (!u) continue;
+    char head[96];
+    snprintf(head, sizeof head, "RUNG %llu %llu %u ", (unsigned long long)p, (unsigned long long)A, r);
+    line = head;
+    append_u128(line, u);
+    line.push_back('\n');
+    return true;
+  }
+  return false;
+}
+
+// rung_scan3.cpp:621-647: n = p + 4a^2 d for a, d <= 8; the least divisor m of n with
+// m == -1 (mod 4ad) whose c = (m+1)/(4ad), k = n/m satisfy ck > a.
+bool solve_direct(u64 p, std::string &line) {
+  for (u64 a = 1; a <= 8; ++a)
+    for (u64 d = 1; d <= 8; ++d) {
+      const u128 nn = (u128)p + (u128)4 * a * a * d;
+      if (nn > kDirectNMax) continue;
+      const u64 n = (u64)nn, q4 = 4 * a * d;
+      Fac f;
+      factor(n, f);
+      if (f.overflow) {
+        fprintf(stderr, "FATAL: more than 16 distinct prime factors\n");
+        abort();
+      }
+      std::vector<u64> divs{1};
+      for (int i = 0; i < f.k; ++i) {
+        const size_t base = divs.size();
+        u64 pe = 1;
+        for (int e = 1; e <= f.e[i]; ++e) {
+          pe *= f.p[i];
+          for (size_t j = 0; j < base; ++j) divs.push_back(divs[j] * pe);
+        }
+      }
+      std::sort(divs.begin(), divs.end());
+      for (u64 m : divs) {
+        if (m % q4 != q4 - 1) continue;
+        const u64 c = (m + 1) / q4, k = n / m;
+        if (!c || !k || (u128)c * k <= a) continue;
+        char buf[160];
+        snprintf(buf, sizeof buf, "SOLVED %llu %llu %llu %llu %llu\n", (unsigned long long)p,
+                 (unsigned long long)a, (unsigned long long)c, (unsigned long long)d,
+                 (unsigned long long)k);
+        line = buf;
+        return true;
+      }
+    }
+  return false;
+}
+
+int certify_prime(u64 p, std::string &line) {
+  if (solve_rung(p, line)) return 0;
+  if (solve_direct(p, line)) return 1;
+  line = "SURVIVOR " + std::to_string(p) + "\n";
+  return 2;
+}
+
+}  // namespace es

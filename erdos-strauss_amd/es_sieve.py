@@ -1,0 +1,228 @@
+"""ctypes binding of include/es_sieve.h (lib/libes_sieve.so), for tests and the bench.
+
+Mirrors the reference's scanner interface at the process level (rung_scan3's flags,
+records and exit codes) as Python calls. There is no CPU fallback: opening a scanner
+without a HIP device raises EsError.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libes_sieve.so")
+BIN_PATH = os.path.join(HERE, "bin", "es_sieve")
+FILTERS = os.path.join(HERE, "tables", "filters")
+F120 = os.path.join(FILTERS, "f_M120120_mmax2000_runtime.txt")
+F204 = os.path.join(FILTERS, "f_M2042040_mmax2000_gpu.txt")
+F204R = os.path.join(FILTERS, "f_M2042040_mmax2000_runtime.txt")
+
+U64 = ctypes.c_uint64
+
+
+class EsConfig(ctypes.Structure):
+    _fields_ = [("wheel_M", U64), ("mmax", U64), ("sieve_P", U64), ("filter_file", ctypes.c_char_p),
+                ("threads", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class EsCounts(ctypes.Structure):
+    _fields_ = [(n, U64) for n in ("positions", "covered", "sieved", "mr", "composite", "rung", "direct",
+                                   "survivors")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class EsTiming(ctypes.Structure):
+    _fields_ = [("plan_ms", ctypes.c_double), ("batch_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("sieve_ms", ctypes.c_double), ("tail_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("tiles", U64), ("survivors", U64), ("primes", U64), ("replays", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class EsError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads lib/libes_sieve.so once (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EsError("libes_sieve.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        sig = {
+            "es_config_default": (None, [P(EsConfig)]),
+            "es_scanner_open": (ctypes.c_int, [P(EsConfig), P(ctypes.c_void_p)]),
+            "es_scanner_close": (None, [ctypes.c_void_p]),
+            "es_last_error": (ctypes.c_char_p, []),
+            "es_check_interval": (ctypes.c_int, [ctypes.c_void_p, U64, U64]),
+            "es_scan_records": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, ctypes.c_int,
+                                               P(ctypes.c_void_p), P(ctypes.c_size_t), P(ctypes.c_size_t),
+                                               P(ctypes.c_int)]),
+            "es_scan_counts": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, P(EsCounts)]),
+            "es_survivors": (ctypes.c_int, [ctypes.c_void_p, U64, U64, P(U64), P(ctypes.c_uint8), ctypes.c_size_t,
+                                            P(ctypes.c_size_t), P(EsCounts)]),
+            "es_primality": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_uint8)]),
+            "es_certify": (ctypes.c_int, [U64, ctypes.c_char_p, ctypes.c_size_t]),
+            "es_set_layout": (ctypes.c_int, [ctypes.c_void_p, U64]),
+            "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),
+            "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "es_choose_layout": (U64, [ctypes.c_void_p, U64]),
+            "es_last_timing": (ctypes.c_int, [ctypes.c_void_p, P(EsTiming)]),
+            "es_wheel_info": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), P(U64), P(U64)]),
+            "es_mark_count": (ctypes.c_int, [ctypes.c_void_p, U64, U64, U64, P(U64), P(U64), P(U64), P(U64)]),
+            "es_lds_peak": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
+            "es_wheel_classes": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_size_t)]),
+            "es_free": (None, [ctypes.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise EsError(lib().es_last_error().decode())
+    return rc
+
+
+def certify(p):
+    """Stage D then E for a prime p on the host: (kind, line)."""
+    buf = ctypes.create_string_buffer(256)
+    kind = _check(lib().es_certify(p, buf, 256))
+    return kind, buf.value.decode()
+
+
+def _u64s(vals):
+    return (U64 * len(vals))(*vals)
+
+
+class Scanner:
+    """One scanner configuration resident on a HIP device (rung_scan3's filter flags)."""
+
+    def __init__(self, wheel=2042040, mmax=10000, sieve=0, filters=None, threads=0, device=0):
+        L = lib()
+        cfg = EsConfig()
+        L.es_config_default(ctypes.byref(cfg))
+        cfg.wheel_M, cfg.mmax, cfg.sieve_P = wheel, mmax, sieve
+        self._ff = filters.encode() if filters else None
+        cfg.filter_file = self._ff
+        cfg.threads, cfg.device = threads, device
+        h = ctypes.c_void_p()
+        _check(L.es_scanner_open(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.wheel = wheel
+
+    def close(self):
+        if self.h:
+            lib().es_scanner_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check_interval(self, lo, hi):
+        return lib().es_check_interval(self.h, lo, hi) == 0
+
+    def records(self, windows, emit_survivors=True):
+        """[(lo, hi)] -> list of record strings (rung_scan3 stdout per window), exit codes."""
+        n = len(windows)
+        lo = _u64s([w[0] for w in windows])
+        hi = _u64s([w[1] for w in windows])
+        text = ctypes.c_void_p()
+        tl = ctypes.c_size_t()
+        rl = (ctypes.c_size_t * max(1, n))()
+        ec = (ctypes.c_int * max(1, n))()
+        _check(lib().es_scan_records(self.h, lo, hi, n, 1 if emit_survivors else 0, ctypes.byref(text),
+                                     ctypes.byref(tl), rl, ec))
+        try:
+            raw = ctypes.string_at(text, tl.value)
+        finally:
+            lib().es_free(text)
+        out, pos = [], 0
+        for i in range(n):
+            out.append(raw[pos:pos + rl[i]].decode())
+            pos += rl[i]
+        return out, [ec[i] for i in range(n)]
+
+    def counts(self, windows):
+        n = len(windows)
+        out = (EsCounts * max(1, n))()
+        _check(lib().es_scan_counts(self.h, _u64s([w[0] for w in windows]), _u64s([w[1] for w in windows]),
+                                    n, out))
+        return [out[i].as_dict() for i in range(n)]
+
+    def survivors(self, lo, hi, cap=1 << 22):
+        """Stage-C survivors of [lo, hi) ascending, their primality flags, and counters."""
+        p = (U64 * cap)()
+        f = (ctypes.c_uint8 * cap)()
+        n = ctypes.c_size_t()
+        c = EsCounts()
+        _check(lib().es_survivors(self.h, lo, hi, p, f, cap, ctypes.byref(n), ctypes.byref(c)))
+        k = min(n.value, cap)
+        return [p[i] for i in range(k)], [f[i] for i in range(k)], c.as_dict()
+
+    def primality(self, vals):
+        n = len(vals)
+        f = (ctypes.c_uint8 * max(1, n))()
+        _check(lib().es_primality(self.h, _u64s(vals), n, f))
+        return [f[i] for i in range(n)]
+
+    def set_layout(self, Mp):
+        _check(lib().es_set_layout(self.h, Mp))
+
+    def set_launch(self, persistent=True, block=256, max_bits=65536):
+        _check(lib().es_set_launch(self.h, 1 if persistent else 0, block, max_bits))
+
+    def set_tuning(self, c_pair, c_mark, c_bit):
+        _check(lib().es_set_tuning(self.h, c_pair, c_mark, c_bit))
+
+    def choose_layout(self, w):
+        return lib().es_choose_layout(self.h, w)
+
+    def timing(self):
+        t = EsTiming()
+        _check(lib().es_last_timing(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def mark_count(self, lo, hi, Mp=0):
+        """Exact kernel work of [lo, hi): marks, bits, (tile, filter) pairs, tiles."""
+        a, b, c, d = U64(), U64(), U64(), U64()
+        _check(lib().es_mark_count(self.h, lo, hi, Mp, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                   ctypes.byref(d)))
+        return {"marks": a.value, "bits": b.value, "pairs": c.value, "tiles": d.value}
+
+    def lds_peak(self):
+        """Measured LDS atomic-AND operations per second."""
+        v = ctypes.c_double()
+        _check(lib().es_lds_peak(self.h, ctypes.byref(v)))
+        return v.value
+
+    def wheel_classes(self):
+        n = ctypes.c_size_t()
+        _check(lib().es_wheel_classes(self.h, None, 0, ctypes.byref(n)))
+        buf = (U64 * n.value)()
+        _check(lib().es_wheel_classes(self.h, buf, n.value, ctypes.byref(n)))
+        return list(buf)
+
+    def wheel_info(self):
+        a, b, c, d = U64(), U64(), U64(), U64()
+        _check(lib().es_wheel_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        return {"lanes": a.value, "wheel_pmin": b.value, "nfilters": c.value, "ncov": d.value}

@@ -1,0 +1,155 @@
+/* es_sieve.h -- C ABI of the MI355X verification-sieve funnel.
+ *
+ * Plain C types only: integers, pointers, sizes. Every entry point returns 0 on success
+ * and a negative status on failure (es_last_error() then says why); nothing aborts
+ * across this boundary. A scanner owns its device memory (tables resident in HBM) and
+ * is used from one host thread at a time.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository):
+ *   sieve/rung_scan3.cpp        the CPU scanner (CLI, record, stage D/E)
+ *   sieve/cuda/scan.cuh:6-20    ScanOpts / scan_run, the GPU driver
+ *   sieve/cuda/ref_host.h       the host accessors the GPU driver binds
+ *   sieve/cuda/mark.cuh,
+ *   sieve/cuda/survivors.cuh    the device stages (k_mark, k_compact, k_primality)
+ */
+#ifndef ES_SIEVE_H
+#define ES_SIEVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ES_OK 0
+#define ES_ERR_CONFIG (-1)   /* a refusal rung_scan3 makes with exit code 1 */
+#define ES_ERR_DEVICE (-2)   /* HIP device missing or a HIP call failed */
+#define ES_ERR_ARG (-3)      /* bad argument to this API */
+
+typedef struct es_scanner es_scanner;
+
+/* Scanner configuration: the flags of rung_scan3.cpp:984-1016 that shape the filter
+ * set, plus the device ordinal and the host-tail thread count. */
+typedef struct es_config {
+  uint64_t wheel_M;          /* --wheel, default 2042040 */
+  uint64_t mmax;             /* --mmax, default 10000 */
+  uint64_t sieve_P;          /* --sieve, default 0 (stage B' off) */
+  const char *filter_file;   /* --filters FILE, or NULL for generated covers */
+  int threads;               /* host tail threads, 0 = all */
+  int device;                /* HIP device ordinal; -1 = host-side planning only (every
+                                call that needs the device then fails ES_ERR_DEVICE) */
+} es_config;
+
+/* SUMMARY counters of one window (rung_scan3.cpp:651-660, :1153-1161). */
+typedef struct es_counts {
+  uint64_t positions, covered, sieved, mr, composite, rung, direct, survivors;
+} es_counts;
+
+/* Timing of the last scan call, milliseconds (scan.cu:150-162's --profile split). */
+typedef struct es_timing {
+  double plan_ms, batch_ms, device_ms, sieve_ms, tail_ms, total_ms;
+  uint64_t tiles, survivors, primes;
+  int replays;
+} es_timing;
+
+/* Fills *cfg with rung_scan3's defaults. */
+void es_config_default(es_config *cfg);
+
+/* Builds the wheel (pin-checked), the filter set and the device context.
+ * Replaces ref3_build_wheel + ref3_filters (sieve/cuda/ref_host.h:39-50) and the
+ * setup half of scan_run (sieve/cuda/scan.cu:55-192). ES_ERR_CONFIG on a pin mismatch
+ * or an unreadable / empty filter file (rung_scan3.cpp:1041-1048, :1085-1097). */
+int es_scanner_open(const es_config *cfg, es_scanner **out);
+void es_scanner_close(es_scanner *s);
+
+/* Message of the last failure on this thread. */
+const char *es_last_error(void);
+
+/* Interval refusals of rung_scan3.cpp:1057-1069 (HI above the stage-E ceiling, LO not
+ * above the filter table's pmin): ES_OK or ES_ERR_CONFIG. */
+int es_check_interval(const es_scanner *s, uint64_t lo, uint64_t hi);
+
+/* The funnel over nwin windows [lo[i], hi[i]) in one device batch, producing each
+ * window's record exactly as rung_scan3 prints it for that interval: certificate
+ * lines in lane order, then SUMMARY. Records are concatenated into *text (free with
+ * es_free); rec_len[i] (optional) receives each record's byte length and exit_code[i]
+ * (optional) 2 if the window has survivors, else 0. Replaces rung_scan3's main scan
+ * (rung_scan3.cpp:1128-1162) and scan_run (sieve/cuda/scan.cu:55-360). */
+int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin,
+                    int emit_survivors, char **text, size_t *text_len, size_t *rec_len,
+                    int *exit_code);
+
+/* As es_scan_records, counters only. */
+int es_scan_counts(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin,
+                   es_counts *out);
+
+/* Stage-C survivors of [lo, hi) (positions no filter removed) with their primality,
+ * in ascending p, plus the window's counters. The parity hook of sieve_cuda's
+ * --check-mark / --check-compact (sieve/cuda/sieve_cuda.cu:30-88). *n receives the
+ * survivor count even when it exceeds cap (then nothing beyond cap is written). */
+int es_survivors(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t *p, uint8_t *is_prime,
+                 size_t cap, size_t *n, es_counts *counts);
+
+/* Deterministic Miller-Rabin on the device for arbitrary values (flags[i] = 1 iff
+ * vals[i] is prime). Replaces k_primality (sieve/cuda/survivors.cuh:47-48) as used by
+ * --check-primality (sieve/cuda/sieve_cuda.cu:89-102). */
+int es_primality(es_scanner *s, const uint64_t *vals, size_t n, uint8_t *flags);
+
+/* Stage D then stage E for one prime p, on the host: writes "RUNG p A r u\n",
+ * "SOLVED p a c d k\n" or "SURVIVOR p\n" into buf and returns 0, 1 or 2 (negative if
+ * buf is too small). Replaces ref3_solve_rung / ref3_solve_direct
+ * (sieve/cuda/ref_host.h:20-21; rung_scan3.cpp:578-647). */
+int es_certify(uint64_t p, char *buf, size_t cap);
+
+/* Sub-wheel layout for the following scans: Mp must divide the wheel modulus; 0
+ * restores the autotuner. No counterpart in the reference (whose only layout is
+ * Mp = M, rung_scan3.cpp:711-716); output does not depend on it. */
+int es_set_layout(es_scanner *s, uint64_t Mp);
+
+/* Launch shape: persistent work-queue kernel (1) or one workgroup per tile (0),
+ * threads per workgroup (multiple of 64, <= 1024), bits per tile (multiple of 32).
+ * Replaces ScanOpts::block (sieve/cuda/scan.cuh:14) and --seg's role as the bitmap
+ * size (rung_scan3.cpp:999). */
+int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bits);
+
+/* Cost-model coefficients of the layout autotuner (see es_set_layout). */
+int es_set_tuning(es_scanner *s, double c_pair, double c_mark, double c_bit);
+
+/* Layout the autotuner picks for a window of w integers. */
+uint64_t es_choose_layout(const es_scanner *s, uint64_t w);
+
+/* Timing of the last es_scan_* / es_survivors call. */
+int es_last_timing(const es_scanner *s, es_timing *t);
+
+/* Wheel facts: lanes (classes), wheel_pmin, filter count. Replaces ref3_wheel_size /
+ * ref3_wheel_pmin (sieve/cuda/ref_host.h:32-34). */
+int es_wheel_info(const es_scanner *s, uint64_t *lanes, uint64_t *wheel_pmin, uint64_t *nfilters,
+                  uint64_t *ncov);
+
+/* Exact work the marking kernel issues for [lo, hi) in layout Mp (0 = the autotuner's
+ * pick): bit clears, bitmap bits, (tile, filter) setups and tiles. Host arithmetic
+ * only. Replaces the exact `marks` count of sieve_cuda --profile
+ * (sieve/cuda/scan.cu:150-162); it is the numerator of the marking roofline. */
+int es_mark_count(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t Mp, uint64_t *marks,
+                  uint64_t *bits, uint64_t *pairs, uint64_t *tiles);
+
+/* Measured LDS atomic-AND rate of the device (32-bit operations per second), the
+ * denominator of the marking roofline. Counterpart of sieve_cuda --bench-mark
+ * (sieve/cuda/mark.cu:197-263), which times the marking kernel itself. */
+int es_lds_peak(es_scanner *s, double *ops_per_s);
+
+/* The wheel classes (ascending), for lane-order merging of window records. Writes
+ * min(cap, lanes) values and returns ES_OK; *n receives the lane count. Replaces
+ * ref3_wheel (sieve/cuda/ref_host.h:33). */
+int es_wheel_classes(const es_scanner *s, uint64_t *out, size_t cap, size_t *n);
+
+/* Frees memory returned by this library. */
+void es_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ES_SIEVE_H */

@@ -1,0 +1,90 @@
+"""CPU checks of the product's native library: it loads, it exports every entry point
+include/es_sieve.h declares, its host-side stage D/E reproduces the reference's golden
+certificate lines, and its CLI's host-only modes match the reference's outputs."""
+import ctypes
+import hashlib
+import os
+import re
+
+import pytest
+
+from tests import paths
+
+pytestmark = pytest.mark.skipif(not os.path.exists(paths.SIEVE_LIB), reason="libes_sieve.so not built")
+
+
+def _declared():
+    src = open(paths.HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(es_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(paths.SIEVE_LIB)
+    names = _declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_open_without_device_fails_loudly():
+    """No CPU fallback: opening a scanner on a machine without a HIP device is an error."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("es_sieve", os.path.join(paths.PKG, "es_sieve.py"))
+    es = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(es)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(es.EsError):
+        es.Scanner(filters=es.F204)
+
+
+GOLDEN_LINE_FILES = ["w19_certificates.txt", "w64_head.txt", "s69_tail.txt", "s0_head.txt"]
+
+
+@pytest.mark.parametrize("fn", GOLDEN_LINE_FILES)
+def test_host_tail_reproduces_golden_lines(fn):
+    """es_certify (Montgomery stage D/E) re-derives the reference's published lines."""
+    lib = ctypes.CDLL(paths.SIEVE_LIB)
+    lib.es_certify.argtypes = [ctypes.c_uint64, ctypes.c_char_p, ctypes.c_size_t]
+    buf = ctypes.create_string_buffer(256)
+    lines = [l for l in open(os.path.join(paths.GOLDEN, "rederive", fn)) if l.startswith("RUNG")]
+    for line in lines:
+        assert lib.es_certify(int(line.split()[1]), buf, 256) == 0
+        assert buf.value.decode() == line
+
+
+def test_cli_self_test_and_class_tables():
+    rc, out, _ = paths.run(paths.SIEVE_BIN, ["--verify"])
+    assert rc == 0 and b"SELF-TEST PASSED" in out
+    want = dict(l.split()[::-1] for l in open(os.path.join(paths.GOLDEN, "class_tables.sha256")))
+    for M in ("120120", "2042040"):
+        rc, out, _ = paths.run(paths.SIEVE_BIN, ["--dump-classes", "--wheel", M])
+        assert rc == 0 and hashlib.sha256(out).hexdigest() == want[M]
+
+
+@pytest.mark.parametrize("lo", [1000003, 9223372036854775808, 18446744065119000000])
+def test_cli_check_direct_rows(lo):
+    rc, out, _ = paths.run(paths.SIEVE_BIN, ["--check-direct", str(lo), "40"])
+    assert rc == 0
+    assert out.decode() == open(os.path.join(paths.GOLDEN, "direct", "%d.txt" % lo)).read()
+
+
+def test_cli_refusals():
+    """Exit code 1 and nothing on stdout for the reference's refusals."""
+    F = os.path.join(paths.FILTERS, "f_M2042040_mmax2000_gpu.txt")
+    for argv in (["1000", "2000"],                                      # LO below the pmin floor
+                 ["1000000000000", "18446744065119614977", "--filters", F],   # HI above the ceiling
+                 ["1000000000000", "1000000001000", "--bogus"],          # unknown option
+                 ["1000000000000", "1000000001000", "--filters", "/nonexistent"]):
+        rc, out, _ = paths.run(paths.SIEVE_BIN, argv)
+        assert rc == 1 and out == b"", argv
+
+
+def test_cli_check_factor():
+    rc, out, _ = paths.run(paths.SIEVE_BIN, ["--check-factor", "40"])
+    assert rc == 0 and out.startswith(b"ok")

@@ -1,0 +1,231 @@
+"""GPU parity: the HIP funnel against the oracle and the reference's records.
+
+Bar: bit-exact. Survivor sets and primality flags must equal the oracle's element for
+element; records (certificate lines in lane order + SUMMARY) and exit codes must be
+byte-identical to the reference scanner's golden records. Every sub-wheel layout and
+launch shape must give the same bytes.
+"""
+import importlib.util
+import os
+import subprocess
+
+import pytest
+
+from tests import paths
+
+pytestmark = pytest.mark.gpu
+
+spec = importlib.util.spec_from_file_location("es_sieve", os.path.join(paths.PKG, "es_sieve.py"))
+es = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(es)
+
+F120, F204 = es.F120, es.F204
+TOP = 18446744065119614976
+
+
+def parse_args(argv):
+    """rung_scan3 argv -> (scanner kwargs, lo, hi, emit_survivors), shard applied."""
+    lo, hi = int(argv[0]), int(argv[1])
+    kw = {"wheel": 2042040, "mmax": 10000, "sieve": 0, "filters": None}
+    shard, nshard, emit = 0, 1, True
+    i = 2
+    while i < len(argv):
+        a = argv[i]
+        if a == "--wheel": kw["wheel"] = int(argv[i + 1]); i += 2
+        elif a == "--mmax": kw["mmax"] = int(argv[i + 1]); i += 2
+        elif a == "--sieve": kw["sieve"] = int(argv[i + 1]); i += 2
+        elif a == "--filters": kw["filters"] = os.path.join(paths.REPO, argv[i + 1]); i += 2
+        elif a == "--seg": i += 2
+        elif a == "--shard":
+            shard, nshard = map(int, argv[i + 1].split("/")); i += 2
+        elif a == "--no-emit-survivors": emit = False; i += 1
+        else: raise ValueError(a)
+    if nshard > 1:
+        span = (hi - lo + nshard - 1) // nshard
+        a = lo + shard * span
+        lo, hi = a, max(a, min(hi, a + span))
+    return kw, lo, hi, emit
+
+
+_scanners = {}
+
+
+def scanner(**kw):
+    key = tuple(sorted(kw.items()))
+    if key not in _scanners:
+        _scanners[key] = es.Scanner(device=0, **kw)
+    sc = _scanners[key]
+    sc.set_layout(0)
+    sc.set_launch(True, 256, 65536)
+    return sc
+
+
+def oracle_survivors(lo, hi, kw):
+    argv = [paths.ORACLE_BIN, str(lo), str(hi), "--wheel", str(kw["wheel"]), "--mmax", str(kw["mmax"]),
+            "--dump-survivors", "--threads", "8"]
+    if kw.get("filters"):
+        argv += ["--filters", kw["filters"]]
+    if kw.get("sieve"):
+        argv += ["--sieve", str(kw["sieve"])]
+    out = subprocess.run(argv, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, check=True).stdout.decode()
+    pairs = sorted((int(t[1]), int(t[2])) for t in (l.split() for l in out.splitlines()))
+    return [p for p, _ in pairs], [f for _, f in pairs]
+
+
+def oracle_record(lo, hi, kw, emit=True):
+    argv = [paths.ORACLE_BIN, str(lo), str(hi), "--wheel", str(kw["wheel"]), "--mmax", str(kw["mmax"]),
+            "--threads", "8"]
+    if kw.get("filters"):
+        argv += ["--filters", kw["filters"]]
+    if kw.get("sieve"):
+        argv += ["--sieve", str(kw["sieve"])]
+    if not emit:
+        argv.append("--no-emit-survivors")
+    p = subprocess.run(argv, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL)
+    return p.stdout.decode(), p.returncode
+
+
+def summary_counts(rec):
+    last = rec.strip().splitlines()[-1].split()
+    return {t.split("=")[0]: int(t.split("=")[1]) for t in last[1:] if "=" in t and t.split("=")[1].isdigit()}
+
+
+# ---------------------------------------------------------------- records
+
+RECORDS = paths.record_cases()
+
+
+@pytest.mark.parametrize("name,argv,rc", RECORDS, ids=[c[0] for c in RECORDS])
+def test_record_via_capi(name, argv, rc):
+    """Every committed reference record, byte for byte, through the C ABI."""
+    kw, lo, hi, emit = parse_args(argv)
+    print("record", name, lo, hi, kw, flush=True)
+    sc = scanner(**kw)
+    (rec,), (code,) = sc.records([(lo, hi)], emit_survivors=emit)
+    assert rec == paths.golden_record(name).decode()
+    assert code == rc
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "sieve1000", "shard3of7", "mmax60_w120_nosurv", "empty"])
+def test_record_via_cli(name):
+    """The drop-in binary: same argv, same stdout bytes, same exit code."""
+    argv, rc = [(a, r) for n, a, r in RECORDS if n == name][0]
+    print("cli", name, argv, flush=True)
+    got_rc, out, err = paths.run(paths.SIEVE_BIN, argv, timeout=300)
+    assert got_rc == rc, err.decode()
+    assert out == paths.golden_record(name)
+
+
+@pytest.mark.parametrize("k", [6, 9, 13, 16, 19])
+def test_decade_records(k):
+    """The reference's own records data/rederive/decades/d<k>.txt."""
+    want = open(os.path.join(paths.GOLDEN, "rederive", "d%d.txt" % k)).read()
+    s = summary_counts(want)
+    print("decade", k, flush=True)
+    sc = scanner(filters=F204)
+    (rec,), _ = sc.records([(s["lo"], s["hi"])])
+    assert rec == want
+
+
+# ---------------------------------------------------------------- survivors (kernel parity)
+
+SURV_CASES = [
+    ("c1win", 255255, 255255 + 2 ** 20, dict(wheel=120120, filters=F120)),
+    ("w120_1e12", 10 ** 12, 10 ** 12 + 3 * 10 ** 8, dict(wheel=120120, filters=F120)),
+    ("w204_1e12", 10 ** 12, 10 ** 12 + 10 ** 9, dict(filters=F204)),
+    ("w204_top", TOP - 2 ** 31, TOP, dict(filters=F204)),
+    ("mmax200", 10 ** 12, 10 ** 12 + 10 ** 8, dict(mmax=200)),
+    ("sieve", 10 ** 12, 10 ** 12 + 10 ** 9, dict(filters=F204, sieve=1000)),
+    ("sieve_w120", 10 ** 11, 10 ** 11 + 10 ** 8, dict(wheel=120120, filters=F120, sieve=3000)),
+]
+
+
+def candidates(M):
+    q = M // 840
+    return sorted(M // d for d in range(1, q + 1) if q % d == 0)
+
+
+@pytest.mark.parametrize("name,lo,hi,kw", SURV_CASES, ids=[c[0] for c in SURV_CASES])
+def test_survivors_every_layout(name, lo, hi, kw):
+    """Stage-C survivor set, primality flags and counters equal the oracle's in every
+    sub-wheel layout (k_mark/k_compact/k_primality parity, sieve_cuda --check-mark)."""
+    full = dict(wheel=2042040, mmax=10000, sieve=0, filters=None)
+    full.update(kw)
+    want_p, want_f = oracle_survivors(lo, hi, full)
+    want_rec, _ = oracle_record(lo, hi, full)
+    want_c = summary_counts(want_rec)
+    sc = scanner(**full)
+    for Mp in [0] + candidates(full["wheel"]):
+        print(name, "layout", Mp, flush=True)
+        sc.set_layout(Mp)
+        p, f, c = sc.survivors(lo, hi)
+        assert p == want_p, (name, Mp)
+        assert f == want_f, (name, Mp)
+        for key in ("positions", "covered", "sieved", "mr", "composite"):
+            assert c[key] == want_c[key], (name, Mp, key)
+
+
+@pytest.mark.parametrize("persistent,block,bits", [(True, 64, 32), (False, 128, 1024), (True, 512, 4096),
+                                                     (False, 1024, 262144), (True, 256, 65536)])
+def test_launch_shapes(persistent, block, bits):
+    """Workgroup size, tile size and the persistent queue do not change a byte."""
+    lo, hi = 10 ** 13, 10 ** 13 + 2 * 10 ** 9
+    kw = dict(wheel=2042040, mmax=10000, sieve=1000, filters=F204)
+    want, _ = oracle_record(lo, hi, kw)
+    sc = scanner(**kw)
+    print("shape", persistent, block, bits, flush=True)
+    for Mp in (840, 2042040):
+        sc.set_layout(Mp)
+        sc.set_launch(persistent, block, bits)
+        (rec,), _ = sc.records([(lo, hi)])
+        assert rec == want, (persistent, block, bits, Mp)
+
+
+def test_survivor_overflow_replay():
+    """More survivors than the initial device list holds: counts are complete, the
+    batch replays with a larger list (scan.cu:223-242), output unchanged."""
+    lo, hi = 10 ** 9, 10 ** 9 + 4 * 10 ** 8
+    kw = dict(wheel=120120, mmax=60, sieve=0, filters=None)
+    want, rc = oracle_record(lo, hi, kw)
+    sc = scanner(**kw)
+    (rec,), (code,) = sc.records([(lo, hi)])
+    t = sc.timing()
+    assert t["survivors"] > 65536
+    assert t["replays"] >= 1
+    assert rec == want and code == rc
+
+
+def test_batch_equals_alone():
+    """A batch of uneven windows gives each window the record it gets alone, and the
+    oracle's record."""
+    wins = [tuple(map(int, l.split())) for l in open(os.path.join(paths.GOLDEN, "c4_windows.txt"))
+            if not l.startswith("#")][:24]
+    kw = dict(wheel=2042040, mmax=10000, sieve=0, filters=F204)
+    sc = scanner(**kw)
+    recs, codes = sc.records(wins)
+    for (lo, hi), rec in zip(wins, recs):
+        print("window", lo, hi, flush=True)
+        (alone,), _ = sc.records([(lo, hi)])
+        assert rec == alone
+        want, _ = oracle_record(lo, hi, kw)
+        assert rec == want
+
+
+# ---------------------------------------------------------------- Miller-Rabin
+
+def test_primality_dense_and_kats():
+    """Device MR against the oracle's test, exhaustively over dense odd ranges at 10^12,
+    2^63 and the ceiling, plus strong pseudoprimes (sieve_cuda --check-primality)."""
+    import ctypes
+    olib = ctypes.CDLL(paths.ORACLE_LIB)
+    olib.es_oracle_is_prime.argtypes = [ctypes.c_uint64]
+    sc = scanner(filters=F204)
+    vals = []
+    for base in (10 ** 12 + 1, 2 ** 63 - 100001, TOP - 200001, 3):
+        vals += [base + 2 * i for i in range(100000)]
+    vals += [2047, 3277, 4033, 4681, 8321, 3215031751, 2152302898747, 3474749660383, 341550071728321,
+             3825123056546413051, 18446744073709551557, 18446744065119614953, 2, 1, 0, 4, 9, 25]
+    got = sc.primality(vals)
+    want = [olib.es_oracle_is_prime(v) for v in vals]
+    bad = [v for v, g, w in zip(vals, got, want) if g != w]
+    assert not bad, bad[:10]
