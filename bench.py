@@ -75,6 +75,63 @@ def cpu_baseline(cores):
             "record_identical": out == gold}
 
 
+def split_records(text):
+    out, cur = [], []
+    for line in text.splitlines(keepends=True):
+        cur.append(line)
+        if line.startswith("SUMMARY "):
+            out.append("".join(cur))
+            cur = []
+    return out
+
+
+def c4_bench(es, reps=5):
+    """Batched configuration: 512 windows (seed 0, start log-uniform over [1e9, 1.8e19],
+    length log-uniform over [2^16, 2^26]). For each wheel the autotuner measures the
+    device on a 64-window sample; the batched form (one persistent launch replayed from
+    a hipGraph) and the plain form (one launch per window) are timed; every window's
+    record must equal the golden record for that wheel. The faster wheel is the choice."""
+    gdir = os.path.join(REPO, "tests", "golden")
+    wins = [tuple(map(int, l.split())) for l in open(os.path.join(gdir, "c4_windows.txt"))
+            if not l.startswith("#")]
+    total = sum(b - a for a, b in wins)
+    res = {}
+    for M, F in ((2042040, es.F204), (120120, es.F120)):
+        gold = split_records(open(os.path.join(gdir, "c4", "records_w%d.txt" % M)).read())
+        sc = es.Scanner(wheel=M, filters=F, device=0)
+        report = sc.autotune(wins[:64])
+        sc.set_graph(True)
+        recs, _ = sc.records(wins)
+        ok = recs == gold
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            recs, _ = sc.records(wins)
+            ts.append(time.perf_counter() - t)
+            ok = ok and recs == gold
+        tm = sc.timing()
+        sc.set_graph(False)
+        block, bits = 256, 32768
+        for l in report.splitlines():
+            if l.startswith("choice block="):
+                block, bits = [int(x.split("=")[1]) for x in l.split()[1:3]]
+        sc.set_launch(False, block, bits)
+        t = time.perf_counter()
+        precs = [sc.records([w])[0][0] for w in wins]
+        plain_s = time.perf_counter() - t
+        ok_plain = precs == gold
+        bt = sorted(ts)[len(ts) // 2]
+        res[M] = {"batched": total / bt, "batched_ms": bt * 1e3, "plain": total / plain_s,
+                  "plain_ms": plain_s * 1e3, "batched_over_plain": plain_s / bt,
+                  "records_identical": ok and ok_plain, "tiles": tm["tiles"], "block": block,
+                  "tile_bits": bits, "device_ms": tm["device_ms"], "tail_ms": tm["tail_ms"],
+                  "layouts": sorted({l.split()[2] for l in report.splitlines() if l.startswith("choice W=")})}
+        sc.close()
+    wheel = max(res, key=lambda m: res[m]["batched"])
+    return {"workload": "512 windows, seed 0, lengths 2^16..2^26, %d integers" % total, "wheel_choice": wheel,
+            "by_wheel": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,12 +139,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched 512-window configuration")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve shards into independent intervals with no "
               "exchange step, and only the one-GPU configuration is built here.", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy:
+        import torch
+        src = torch.ones(args.calib_copy << 18, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     es = load("es_sieve")
     records = load("records")
     gold = open(os.path.join(REPO, "tests", "golden", "records", "c2.txt")).read()
@@ -118,17 +185,21 @@ def main():
     # ---- plain form: one launch per 2^22-integer window
     wins = [(a, min(HI, a + WINDOW)) for a in range(LO, HI, WINDOW)]
     classes = sc.wheel_classes()
-    sc.set_launch(False, 256, 65536)
-    (r0,), _ = sc.records([wins[0]])          # warm the window layout
-    plain_wall = []
-    ok_plain = True
-    for _ in range(args.plain_steps):
-        t = time.perf_counter()
-        recs = [sc.records([w])[0][0] for w in wins]
-        plain_wall.append(time.perf_counter() - t)
-        ok_plain = ok_plain and records.merge(recs, classes, LO, HI) == gold
-    sc.set_launch(True, 256, 65536)
-    plain_value = (HI - LO) * len(plain_wall) / sum(plain_wall)
+    plain = None
+    if args.plain_steps > 0:
+        sc.set_launch(False, 256, 32768)
+        (r0,), _ = sc.records([wins[0]])          # warm the window layout
+        plain_wall = []
+        ok_plain = True
+        for _ in range(args.plain_steps):
+            t = time.perf_counter()
+            recs = [sc.records([w])[0][0] for w in wins]
+            plain_wall.append(time.perf_counter() - t)
+            ok_plain = ok_plain and records.merge(recs, classes, LO, HI) == gold
+        sc.set_launch(True, 256, 32768)
+        plain_value = (HI - LO) * len(plain_wall) / sum(plain_wall)
+        plain = {"value": plain_value, "window": WINDOW, "windows": len(wins),
+                 "s_per_pass": sum(plain_wall) / len(plain_wall), "record_identical": ok_plain}
 
     # ---- roofline of the marking kernel (k_sieve)
     lds_ops = sc.lds_peak()
@@ -151,14 +222,13 @@ def main():
         "config": {"workload": "plain sieve at height 1e15: [1e15, 1e15+2^34), wheel 2042040, "
                                "f_M2042040_mmax2000_gpu.txt (3745 filters)",
                    "form": "batched (persistent work queue, autotuned layout)", "layout_Mp": layout,
-                   "tiles": work["tiles"], "block": 256, "tile_bits": 65536},
+                   "tiles": work["tiles"], "block": 256, "tile_bits": 32768},
         "record_identical": ok_batched,
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
         "split_ms": {"device": sorted(device_ms)[len(device_ms) // 2], "sieve_kernel": kernel_s * 1e3,
                      "host_tail": timing["tail_ms"], "plan": timing["plan_ms"], "batch": timing["batch_ms"]},
-        "plain": {"value": plain_value, "window": WINDOW, "windows": len(wins),
-                  "s_per_pass": sum(plain_wall) / len(plain_wall), "record_identical": ok_plain},
-        "batched_over_plain": value / plain_value,
+        "plain": plain,
+        "batched_over_plain": value / plain["value"] if plain else None,
         "roofline": {"bound": "lds", "achieved": achieved, "peak": NOMINAL_LDS_BPS / 1e9, "unit": "GB/s",
                      "frac": achieved / (NOMINAL_LDS_BPS / 1e9),
                      "peak_measured": lds_ops * 8 / 1e9,
@@ -166,6 +236,8 @@ def main():
                      "marks": work["marks"], "bits": work["bits"], "pairs": work["pairs"],
                      "kernel_ms": kernel_s * 1e3, "traffic": None},
     }
+    if not args.no_c4:
+        out["c4"] = c4_bench(es)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(os.cpu_count() or 1, 16))
     print(json.dumps(out))

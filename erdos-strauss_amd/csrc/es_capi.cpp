@@ -157,12 +157,18 @@ int es_set_layout(es_scanner *s, uint64_t Mp) {
 
 int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bits) {
   if (!s) return fail(ES_ERR_ARG, "null scanner");
-  if (block == 0 || block > 1024 || block % 64 || max_bits == 0 || max_bits % 32 || max_bits > (1u << 18))
+  if (block == 0 || block > 1024 || block % 64 || max_bits == 0 || max_bits % 32 || max_bits > es::kMaxTileBits)
     return fail(ES_ERR_ARG, "bad launch shape");
   es::LaunchCfg &c = s->sc.tuning().launch;
   c.persistent = persistent != 0;
   c.block = block;
   c.max_bits = max_bits;
+  return ES_OK;
+}
+
+int es_set_graph(es_scanner *s, int on) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  s->sc.tuning().launch.graph = on != 0;
   return ES_OK;
 }
 
@@ -172,6 +178,23 @@ int es_set_tuning(es_scanner *s, double c_pair, double c_mark, double c_bit) {
   t.c_pair = c_pair;
   t.c_mark = c_mark;
   t.c_bit = c_bit;
+  return ES_OK;
+}
+
+int es_autotune(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, char **report) {
+  if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
+  std::vector<es::Interval> w(nwin);
+  std::string msg, rep;
+  for (size_t i = 0; i < nwin; ++i) {
+    w[i] = es::Interval{lo[i], hi[i]};
+    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
+  }
+  if (s->sc.autotune(w, rep, msg)) return fail(ES_ERR_DEVICE, msg);
+  if (report) {
+    *report = (char *)malloc(rep.size() + 1);
+    if (!*report) return fail(ES_ERR_ARG, "out of host memory");
+    memcpy(*report, rep.c_str(), rep.size() + 1);
+  }
   return ES_OK;
 }
 

@@ -39,11 +39,15 @@ struct BatchResult {
   int replays = 0;                    // survivor-buffer overflows that forced a rerun
 };
 
+// Largest tile: one LDS byte per position, within the 64 KiB a workgroup gets by
+// default (the rest of the kernel's LDS is a few hundred bytes).
+constexpr u32 kMaxTileBits = 61440;
+
 struct LaunchCfg {
   bool persistent = true;   // work-queue persistent kernel vs one workgroup per tile
   bool graph = false;       // replay a captured hipGraph (persistent form only)
   u32 block = 256;          // threads per workgroup
-  u32 max_bits = 65536;     // largest tile (LDS bitmap = max_bits/8 bytes)
+  u32 max_bits = 32768;     // largest tile in positions (LDS bytes of the byte map)
 };
 
 class DeviceCtx {

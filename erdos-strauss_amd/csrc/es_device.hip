@@ -39,6 +39,7 @@ struct LayoutDev {
   const u32 *step;
   const u32 *magic;
   const u32 *c32;
+  const uint4 *fc;           // per ordered filter: {s, floor(2^32/s), 2^32 mod s, 0}
   const u16 *off;
   u32 P, Pmagic, Pc32, nl, nf, ncov, cov_small, pr_small, mask_words, pad;
 };
@@ -64,49 +65,45 @@ __device__ __forceinline__ u32 mod64(u64 x, u32 s, u32 mg, u32 c32) {
   return t >= s ? t - s : t;
 }
 
-// Bits 0, s, 2s, ... of a 32-bit word (s < 32).
-__device__ __forceinline__ u32 stride_pattern(u32 s) {
-  u32 p = 0;
-  for (u32 i = 0; i < 32; i += s) p |= 1u << i;
-  return p;
-}
-
-// Clears, in the words this thread owns (w = tid, tid + bs, ...), every bit of the
-// progression k == o (mod s), s < 32, for ordered filters [from, to).
-__device__ __forceinline__ void apply_small(u32 *bits, u32 nw, const LayoutDev *L, const u16 *offrow,
-                                            u64 k0, u32 from, u32 to, u32 tid, u32 bs) {
-  for (u32 j = from; j < to; ++j) {
-    const u32 o = offrow[j];
-    if (o == 0xFFFFu) continue;
-    const u32 s = L->step[j], mg = L->magic[j];
-    const u32 km = mod64(k0, s, mg, L->c32[j]);
-    const u32 base = o >= km ? o - km : o + s - km;        // bit index of the first hit, mod s
-    const u32 d0 = mod32(32u * tid, s, mg);
-    const u32 dstep = mod32(32u * bs, s, mg);
-    u32 r = base >= d0 ? base - d0 : base + s - d0;         // first hit inside word tid
-    const u32 pat = stride_pattern(s);
-    for (u32 w = tid; w < nw; w += bs) {
-      bits[w] &= ~(pat << r);
-      r = r >= dstep ? r - dstep : r + s - dstep;
-    }
+// Clears map bytes i, i + s, i + 2s, ... below nb (four stores per bound check).
+__device__ __forceinline__ void clear_run(u8 *map, u32 i, u32 s, u32 nb) {
+  for (; i + 3u * s < nb; i += 4u * s) {
+    map[i] = 0;
+    map[i + s] = 0;
+    map[i + 2u * s] = 0;
+    map[i + 3u * s] = 0;
   }
+  for (; i < nb; i += s) map[i] = 0;
 }
 
-// Clears k == o (mod s) for ordered filters [from, to) with one thread per filter;
-// consecutive rounds reverse the thread order so sorted strides balance.
-__device__ __forceinline__ void apply_scatter(u32 *bits, u32 nb, const LayoutDev *L, const u16 *offrow,
-                                              u64 k0, u32 from, u32 to, u32 tid, u32 bs) {
-  u32 round = 0;
-  for (u32 base = from; base < to; base += bs, ++round) {
-    const u32 j = base + ((round & 1) ? bs - 1 - tid : tid);
+// Marks ordered filters [from, to), one filter per thread. Filters are sorted by
+// ascending stride, so 64 consecutive ones ("a bin") have similar hit counts; waves pull
+// bins from an LDS counter, heaviest first, which balances the workgroup without any
+// host planning.
+__device__ __forceinline__ void mark_range(u8 *map, u32 nb, const LayoutDev *L, const u16 *offrow, u64 k0,
+                                           u32 from, u32 to, u32 *bin_ctr, u32 lane64) {
+  const u32 nbins = (to - from + 63u) >> 6;
+  for (;;) {
+    u32 b = 0;
+    if (lane64 == 0) b = atomicAdd(bin_ctr, 1u);
+    b = __shfl(b, 0, 64);
+    if (b >= nbins) break;
+    const u32 j = from + (b << 6) + lane64;
     if (j >= to) continue;
     const u32 o = offrow[j];
     if (o == 0xFFFFu) continue;
-    const u32 s = L->step[j];
-    const u32 km = mod64(k0, s, L->magic[j], L->c32[j]);
-    u32 i = o >= km ? o - km : o + s - km;
-    for (; i < nb; i += s) atomicAnd(&bits[i >> 5], ~(1u << (i & 31)));
+    const uint4 f = L->fc[j];                       // {s, magic, 2^32 mod s}
+    const u32 km = mod64(k0, f.x, f.y, f.z);
+    clear_run(map, o >= km ? o - km : o + f.x - km, f.x, nb);
   }
+}
+
+// Alive positions of one 32-position chunk as a bit word (map bytes are 0 or 1).
+__device__ __forceinline__ u32 chunk_bits(const u32 *map32, u32 c) {
+  u32 w = 0;
+#pragma unroll
+  for (u32 q = 0; q < 8; ++q) w |= ((((map32[c * 8 + q] & 0x01010101u) * 0x01020408u) >> 24) & 0xFu) << (4 * q);
+  return w;
 }
 
 __device__ __forceinline__ u32 wave_sum(u32 v) {
@@ -114,16 +111,24 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
   return v;
 }
 
+// The tile is a byte map in LDS: byte k says whether position k0 + k is still alive.
+// A mark is a plain byte store of 0 -- idempotent, so concurrent stores need no atomics.
+// All LDS is one dynamic allocation: the map at offset 0 (so a mark's address is its
+// position), then a few scalars. lds_words counts the map in 32-bit words.
 template <bool kPersistent>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
                                                 const LayoutDev *__restrict__ lays,
                                                 u64 *__restrict__ counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
-                                                u32 *__restrict__ surv_count, u32 *__restrict__ queue) {
-  extern __shared__ u32 bits[];
-  __shared__ u32 s_tile, s_base;
-  __shared__ u32 s_wpos[kMaxWaves], s_waft[kMaxWaves], s_wsur[kMaxWaves];
+                                                u32 *__restrict__ surv_count, u32 *__restrict__ queue,
+                                                u32 lds_words) {
+  extern __shared__ u32 lds[];
+  u32 *map32 = lds;
+  u8 *map = (u8 *)lds;
+  u32 *sc = lds + lds_words;   // [0] tile, [1] base, [2] bin counter B, [3] bin counter B',
+                               // [16..) per-wave partial sums (3 x kMaxWaves)
+  u32 *s_wpos = sc + 16, *s_waft = sc + 16 + kMaxWaves, *s_wsur = sc + 16 + 2 * kMaxWaves;
   const u32 tid = threadIdx.x, bs = blockDim.x;
   const u32 lane64 = tid & 63u, wave = tid >> 6, nwaves = (bs + 63u) >> 6;
   const u32 ntiles = hdr->ntiles;
@@ -131,49 +136,48 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   u32 t = blockIdx.x;
   for (;;) {
     if (kPersistent) {
-      if (tid == 0) s_tile = atomicAdd(queue, 1u);
+      if (tid == 0) sc[0] = atomicAdd(queue, 1u);
       __syncthreads();
-      t = s_tile;
+      t = sc[0];
     }
     if (t >= ntiles) break;
     const TileDesc td = tiles[t];
     const LayoutDev *L = lays + td.layout;
-    const u32 nb = td.nbits, nw = (nb + 31u) >> 5;
+    const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
     const u16 *offrow = L->off + (size_t)td.lane * L->nf;
     const u32 P = L->P;
+    if (tid == 0) { sc[2] = 0; sc[3] = 0; }
 
-    // ---- init: wheel mask, tail mask, and the word-wise certified filters
+    // ---- init: wheel mask and tail mask, expanded to one byte per position
     u32 my_pos = 0;
     {
       const u32 *mrow = L->mask + (size_t)td.lane * L->mask_words;
       const u32 kP = P == 1 ? 0u : mod64(td.k0, P, L->Pmagic, L->Pc32);
-      for (u32 w = tid; w < nw; w += bs) {
+      for (u32 c = tid; c < nc; c += bs) {
         u32 word = 0xFFFFFFFFu;
         if (P != 1) {
-          const u32 b = mod32(kP + 32u * w, P, L->Pmagic);
+          const u32 b = mod32(kP + 32u * c, P, L->Pmagic);
           word = __builtin_amdgcn_alignbit(mrow[(b >> 5) + 1], mrow[b >> 5], b & 31u);
         }
-        if (w == nw - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
+        if (c == nc - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
         my_pos += __popc(word);
-        bits[w] = word;
+#pragma unroll
+        for (u32 q = 0; q < 8; ++q) map32[c * 8 + q] = (((word >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u;
       }
     }
-    apply_small(bits, nw, L, offrow, td.k0, 0, L->cov_small, tid, bs);
     __syncthreads();
     // ---- stage B: certified filters
-    apply_scatter(bits, nb, L, offrow, td.k0, L->cov_small, L->ncov, tid, bs);
+    mark_range(map, nb, L, offrow, td.k0, 0, L->ncov, sc + 2, lane64);
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
-      for (u32 w = tid; w < nw; w += bs) my_after += __popc(bits[w]);
+      for (u32 c = tid; c < nc; c += bs) my_after += __popc(chunk_bits(map32, c));
       // ---- stage B': small primes
-      apply_small(bits, nw, L, offrow, td.k0, L->ncov, L->ncov + L->pr_small, tid, bs);
-      __syncthreads();
-      apply_scatter(bits, nb, L, offrow, td.k0, L->ncov + L->pr_small, L->nf, tid, bs);
+      mark_range(map, nb, L, offrow, td.k0, L->ncov, L->nf, sc + 3, lane64);
       __syncthreads();
     }
     u32 my_surv = 0;
-    for (u32 w = tid; w < nw; w += bs) my_surv += __popc(bits[w]);
+    for (u32 c = tid; c < nc; c += bs) my_surv += __popc(chunk_bits(map32, c));
     if (!sieve) my_after = my_surv;
 
     // ---- reductions and the survivor scan (inclusive within the wave)
@@ -198,19 +202,19 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       if (tot_p) atomicAdd((unsigned long long *)&c[0], (unsigned long long)tot_p);
       if (tot_a) atomicAdd((unsigned long long *)&c[1], (unsigned long long)tot_a);
       if (tot_s) atomicAdd((unsigned long long *)&c[2], (unsigned long long)tot_s);
-      s_base = tot_s ? atomicAdd(surv_count, tot_s) : 0u;
+      sc[1] = tot_s ? atomicAdd(surv_count, tot_s) : 0u;
     }
     __syncthreads();
     if (my_surv) {
-      u32 idx = s_base + before + incl - my_surv;
+      u32 idx = sc[1] + before + incl - my_surv;
       const u64 c = L->lanes[td.lane], Mp = L->Mp;
-      for (u32 w = tid; w < nw; w += bs) {
-        u32 x = bits[w];
+      for (u32 ch = tid; ch < nc; ch += bs) {
+        u32 x = chunk_bits(map32, ch);
         while (x) {
           const u32 i = __builtin_ctz(x);
           x &= x - 1;
           if (idx < cap) {
-            surv_p[idx] = c + Mp * (td.k0 + 32ull * w + i);
+            surv_p[idx] = c + Mp * (td.k0 + 32ull * ch + i);
             surv_w[idx] = td.win;
           }
           ++idx;
@@ -218,14 +222,16 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       }
     }
     if (!kPersistent) break;
-    __syncthreads();   // the next tile reuses the bitmap and s_tile
+    __syncthreads();   // the next tile reuses the map and the scalars
   }
 }
 
-__global__ void k_reset(u64 *counts, u32 n, u32 *surv_count, u32 *queue) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) counts[i] = 0;
-  if (i == 0) { *surv_count = 0; *queue = 0; }
+// Zeroes the batch's counters. Sizes come from the device header, so a captured graph
+// of reset + sieve + MR serves every batch that fits the buffers.
+__global__ void k_reset(const BatchHdr *__restrict__ hdr, u64 *counts, u32 *surv_count, u32 *queue) {
+  const u32 n = hdr->nwin * 3;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counts[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { *surv_count = 0; *queue = 0; }
 }
 
 // ------------------------------------------------------------------ Miller-Rabin
@@ -269,32 +275,75 @@ __device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
 
 __constant__ u64 c_small_p[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
 
-__device__ bool is_prime_dev(u64 n) {
-  if (n < 2) return false;
-  if (!(n & 1)) return n == 2;
-  for (int i = 1; i < 12; ++i) {
-    const u64 q = c_small_p[i];
-    if (n == q) return true;
-    if (n % q == 0) return false;
+// q^{-1} mod 2^64 and floor((2^64-1)/q) for the odd primes above: n is divisible by q
+// iff n * q^{-1} (mod 2^64) <= floor((2^64-1)/q), one multiply instead of a division.
+struct DivTest { u64 inv, lim; };
+constexpr u64 inv_pow2(u64 a) {
+  u64 x = a;
+  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+  return x;
+}
+__constant__ DivTest c_div[11] = {
+    {inv_pow2(3), ~0ull / 3},   {inv_pow2(5), ~0ull / 5},   {inv_pow2(7), ~0ull / 7},
+    {inv_pow2(11), ~0ull / 11}, {inv_pow2(13), ~0ull / 13}, {inv_pow2(17), ~0ull / 17},
+    {inv_pow2(19), ~0ull / 19}, {inv_pow2(23), ~0ull / 23}, {inv_pow2(29), ~0ull / 29},
+    {inv_pow2(31), ~0ull / 31}, {inv_pow2(37), ~0ull / 37}};
+
+// Verdict codes written by the two Miller-Rabin passes.
+constexpr u8 kComposite = 0, kPrime = 1, kPending = 2;
+
+// Pass 1, one thread per value: the trial prologue by the first twelve primes and the
+// base-2 test. Values that pass are marked pending and keep 2^64 mod n for pass 2.
+__global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
+                           u8 *__restrict__ flags, u64 *__restrict__ one_out) {
+  const u32 n_ = min(*count, cap);
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n_; i += gridDim.x * blockDim.x) {
+    const u64 n = vals[i];
+    u8 v;
+    if (n < 2) v = kComposite;
+    else if (!(n & 1)) v = n == 2 ? kPrime : kComposite;
+    else if (n <= 37) {
+      v = kComposite;
+      for (int k = 1; k < 12; ++k) if (n == c_small_p[k]) v = kPrime;
+    } else {
+      bool divisible = false;
+      for (int k = 0; k < 11; ++k) divisible |= n * c_div[k].inv <= c_div[k].lim;
+      if (divisible) v = kComposite;
+      else {
+        u64 ninv = n;                         // n^{-1} mod 2^64 (Newton), then negate
+        for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
+        ninv = 0 - ninv;
+        const u64 one = (0 - n) % n;          // 2^64 mod n
+        u64 d = n - 1;
+        const u32 s = (u32)__builtin_ctzll(d);
+        d >>= s;
+        v = strong_probable(n, ninv, one, 2, d, s) ? kPending : kComposite;
+        one_out[i] = one;
+      }
+    }
+    flags[i] = v;
   }
-  u64 ninv = n;                               // n^{-1} mod 2^64 (Newton), then negate
-  for (int i = 0; i < 6; ++i) ninv *= 2 - n * ninv;
-  ninv = 0 - ninv;
-  const u64 one = (0 - n) % n;                // 2^64 mod n
-  u64 d = n - 1;
-  const u32 s = (u32)__builtin_ctzll(d);
-  d >>= s;
-  if (!strong_probable(n, ninv, one, 2, d, s)) return false;   // the cheap filter first
-  for (int i = 1; i < 12; ++i)
-    if (!strong_probable(n, ninv, one, c_small_p[i], d, s)) return false;
-  return true;
 }
 
-__global__ void k_mr(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
-                     u8 *__restrict__ flags) {
-  const u32 n = min(*count, cap);
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    flags[i] = is_prime_dev(vals[i]) ? 1 : 0;
+// Pass 2, one thread per (value, base) for the eleven odd prime bases: any failing
+// base writes the composite verdict (all writers write the same byte value). Pending
+// values that survive every base are prime.
+__global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
+                           u8 *__restrict__ flags, const u64 *__restrict__ one_in) {
+  const u32 n_ = min(*count, cap);
+  const u32 total = n_ * 11u;
+  for (u32 g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const u32 i = g / 11u, b = g - i * 11u + 1u;
+    if (flags[i] != kPending) continue;
+    const u64 n = vals[i];
+    u64 ninv = n;
+    for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
+    ninv = 0 - ninv;
+    u64 d = n - 1;
+    const u32 s = (u32)__builtin_ctzll(d);
+    d >>= s;
+    if (!strong_probable(n, ninv, one_in[i], c_small_p[b], d, s)) flags[i] = kComposite;
+  }
 }
 
 // LDS atomic-AND throughput probe: every thread clears bits of its own word for `iters`
@@ -330,12 +379,16 @@ __global__ __launch_bounds__(256) void k_lds_peak(u32 iters, u32 *sink) {
 struct DevLayoutBufs {
   dev::LayoutDev d;
   void *lanes = nullptr, *mask = nullptr, *step = nullptr, *magic = nullptr, *c32 = nullptr,
-       *off = nullptr;
+       *fc = nullptr, *off = nullptr;
 };
 
 struct DeviceCtx::Impl {
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_ok = false;
+  u32 graph_block = 0, graph_bits = 0;
   std::vector<DevLayoutBufs> layouts;
   dev::LayoutDev *d_lays = nullptr;
   size_t lays_cap = 0;
@@ -348,6 +401,7 @@ struct DeviceCtx::Impl {
   u64 *d_surv_p = nullptr;
   u32 *d_surv_w = nullptr;
   uint8_t *d_flags = nullptr;
+  u64 *d_one = nullptr;       // 2^64 mod n per survivor, from the base-2 pass
   size_t surv_cap = 0;
   u32 *d_scalars = nullptr;   // [0] survivor count, [1] queue head
   u32 *h_count = nullptr;
@@ -364,13 +418,16 @@ DeviceCtx::~DeviceCtx() {
     (void)hipSetDevice(device_);
     if (m.stream) (void)hipStreamSynchronize(m.stream);
     for (DevLayoutBufs &b : m.layouts)
-      for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.off}) if (p) (void)hipFree(p);
+      for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
     for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_counts,
-                    (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_scalars})
+                    (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one,
+                    (void *)m.d_scalars})
       if (p) (void)hipFree(p);
     if (m.h_hdr) (void)hipHostFree(m.h_hdr);
     if (m.h_count) (void)hipHostFree(m.h_count);
-    for (hipEvent_t e : {m.ev0, m.ev1, m.ev2}) if (e) (void)hipEventDestroy(e);
+    if (m.graph_exec) (void)hipGraphExecDestroy(m.graph_exec);
+    if (m.graph) (void)hipGraphDestroy(m.graph);
+    for (hipEvent_t e : {m.ev0, m.ev1, m.ev2, m.ev3}) if (e) (void)hipEventDestroy(e);
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
   delete impl_;
@@ -396,6 +453,7 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipEventCreate(&m.ev0));
   HIP_TRY(hipEventCreate(&m.ev1));
   HIP_TRY(hipEventCreate(&m.ev2));
+  HIP_TRY(hipEventCreate(&m.ev3));
   HIP_TRY(hipMalloc(&m.d_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
@@ -418,6 +476,9 @@ int DeviceCtx::add_layout(const Layout &L) {
       up(&b.step, L.step.data(), L.step.size() * 4) || up(&b.magic, L.magic.data(), L.magic.size() * 4) ||
       up(&b.c32, L.c32.data(), L.c32.size() * 4) || up(&b.off, L.off.data(), L.off.size() * 2))
     return -1;
+  std::vector<uint4> fc(L.nf);
+  for (u32 j = 0; j < L.nf; ++j) fc[j] = make_uint4(L.step[j], L.magic[j], L.c32[j], 0u);
+  if (up(&b.fc, fc.data(), fc.size() * sizeof(uint4))) return -1;
   dev::LayoutDev &d = b.d;
   d.Mp = L.Mp;
   d.lanes = (const u64 *)b.lanes;
@@ -425,6 +486,7 @@ int DeviceCtx::add_layout(const Layout &L) {
   d.step = (const u32 *)b.step;
   d.magic = (const u32 *)b.magic;
   d.c32 = (const u32 *)b.c32;
+  d.fc = (const uint4 *)b.fc;
   d.off = (const u16 *)b.off;
   d.P = (u32)L.P;
   d.Pmagic = L.P == 1 ? 0xFFFFFFFFu : (u32)((1ull << 32) / L.P);
@@ -461,7 +523,8 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   out.surv_prime.clear();
   out.replays = 0;
   out.device_ms = out.sieve_ms = 0;
-  if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32) {
+  if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
+      cfg.max_bits > kMaxTileBits) {
     err_ = "bad launch configuration";
     return -1;
   }
@@ -474,27 +537,33 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     }
   }
   if (nt == 0) return 0;
-  // Capacities.
+  if (nt > 0xFFFFFFFFull || nwin > 0x55555555u) { err_ = "batch too large"; return -1; }
+  // Capacities. Any reallocation invalidates a captured graph.
   if (nt > m.tiles_cap) {
     if (m.d_tiles) HIP_TRY(hipFree(m.d_tiles));
     m.tiles_cap = std::max<size_t>(nt, 2 * m.tiles_cap);
     HIP_TRY(hipMalloc(&m.d_tiles, m.tiles_cap * sizeof(TileDesc)));
+    m.graph_ok = false;
   }
   if (nwin > m.win_cap) {
     if (m.d_counts) HIP_TRY(hipFree(m.d_counts));
     m.win_cap = std::max<size_t>(nwin, 2 * m.win_cap);
     HIP_TRY(hipMalloc(&m.d_counts, m.win_cap * 3 * sizeof(u64)));
+    m.graph_ok = false;
   }
   auto grow_surv = [&](size_t need) -> int {
     if (need <= m.surv_cap) return 0;
     size_t cap = std::max<size_t>(m.surv_cap ? m.surv_cap : (1u << 16), 1u << 16);
     while (cap < need) cap *= 2;
-    if (cap > 0xFFFFFFFFull) { err_ = "survivor count exceeds 2^32"; return -1; }
-    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags}) if (p) HIP_TRY(hipFree(p));
+    if (cap > (1ull << 28)) { err_ = "survivor count exceeds 2^28 per batch"; return -1; }
+    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one})
+      if (p) HIP_TRY(hipFree(p));
     HIP_TRY(hipMalloc(&m.d_surv_p, cap * sizeof(u64)));
+    HIP_TRY(hipMalloc(&m.d_one, cap * sizeof(u64)));
     HIP_TRY(hipMalloc(&m.d_surv_w, cap * sizeof(u32)));
     HIP_TRY(hipMalloc(&m.d_flags, cap));
     m.surv_cap = cap;
+    m.graph_ok = false;
     return 0;
   };
   if (grow_surv(1u << 16)) return -1;
@@ -506,43 +575,78 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   HIP_TRY(hipMemcpyAsync(m.d_hdr, m.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(m.d_tiles, tiles.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
 
-  const size_t lds = cfg.max_bits / 8;
-  if (m.occ_block != cfg.block || m.occ_bits != cfg.max_bits) {
+  // One LDS byte per position, sized to the batch's largest tile (rounded to whole
+  // 32-position chunks): short tiles then leave room for more resident workgroups.
+  u32 tile_max = 0;
+  for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
+  const u32 lds_bytes = (tile_max + 31u) & ~31u;
+  const u32 lds_words = lds_bytes / 4;
+  const size_t lds = lds_bytes + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
+  if (m.occ_block != cfg.block || m.occ_bits != lds_bytes) {
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, lds));
     m.occupancy_blocks = std::max(1, nb);
     m.occ_block = cfg.block;
-    m.occ_bits = cfg.max_bits;
+    m.occ_bits = lds_bytes;
+    m.graph_ok = false;
   }
-  for (;;) {
-    const u32 ncnt = nwin * 3;
-    dev::k_reset<<<(ncnt + 255) / 256 ? (ncnt + 255) / 256 : 1, 256, 0, m.stream>>>(
-        m.d_counts, ncnt, m.d_scalars, m.d_scalars + 1);
+  const u32 pgrid = (u32)((size_t)cus_ * m.occupancy_blocks);
+  // The device part of one batch: reset, sieve, Miller-Rabin. The persistent grid does
+  // not depend on the batch, which is what lets a graph of it be replayed.
+  auto enqueue = [&](bool timed) -> int {
+    dev::k_reset<<<64, 256, 0, m.stream>>>(m.d_hdr, m.d_counts, m.d_scalars, m.d_scalars + 1);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(m.ev0, m.stream));
+    if (timed) HIP_TRY(hipEventRecord(m.ev1, m.stream));
     if (cfg.persistent) {
-      const u32 grid = (u32)std::min<size_t>(nt, (size_t)cus_ * m.occupancy_blocks);
-      dev::k_sieve<true><<<grid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
-                                                             m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                             m.d_scalars, m.d_scalars + 1);
+      dev::k_sieve<true><<<pgrid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
+                                                              m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
+                                                              m.d_scalars, m.d_scalars + 1, lds_words);
     } else {
       dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
                                                                  m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                                 m.d_scalars, m.d_scalars + 1);
+                                                                 m.d_scalars, m.d_scalars + 1, lds_words);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(m.ev1, m.stream));
-    dev::k_mr<<<std::max(1, cus_ * 2), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
-                                                            m.d_flags);
+    if (timed) HIP_TRY(hipEventRecord(m.ev2, m.stream));
+    dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+                                                                  m.d_flags, m.d_one);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(m.ev2, m.stream));
+    dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+                                                                  m.d_flags, m.d_one);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  };
+  const bool use_graph = cfg.graph && cfg.persistent;
+  for (;;) {
+    HIP_TRY(hipEventRecord(m.ev0, m.stream));
+    if (use_graph) {
+      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds_bytes) {
+        if (m.graph_exec) { HIP_TRY(hipGraphExecDestroy(m.graph_exec)); m.graph_exec = nullptr; }
+        if (m.graph) { HIP_TRY(hipGraphDestroy(m.graph)); m.graph = nullptr; }
+        HIP_TRY(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m.stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) { err_ = std::string("hipStreamEndCapture: ") + hipGetErrorString(e); return -1; }
+        m.graph = g;
+        HIP_TRY(hipGraphInstantiate(&m.graph_exec, m.graph, nullptr, nullptr, 0));
+        m.graph_ok = true;
+        m.graph_block = cfg.block;
+        m.graph_bits = lds_bytes;
+      }
+      HIP_TRY(hipGraphLaunch(m.graph_exec, m.stream));
+    } else {
+      if (enqueue(true)) return -1;
+    }
+    HIP_TRY(hipEventRecord(m.ev3, m.stream));
     HIP_TRY(hipMemcpyAsync(m.h_count, m.d_scalars, sizeof(u32), hipMemcpyDeviceToHost, m.stream));
     HIP_TRY(hipStreamSynchronize(m.stream));
-    float a = 0, b = 0;
-    HIP_TRY(hipEventElapsedTime(&a, m.ev0, m.ev1));
-    HIP_TRY(hipEventElapsedTime(&b, m.ev0, m.ev2));
-    out.sieve_ms += a;
-    out.device_ms += b;
+    float all = 0, sv = 0;
+    HIP_TRY(hipEventElapsedTime(&all, m.ev0, m.ev3));
+    if (!use_graph) HIP_TRY(hipEventElapsedTime(&sv, m.ev1, m.ev2));
+    out.device_ms += all;
+    out.sieve_ms += use_graph ? -1.0 : sv;
     const u32 cnt = *m.h_count;
     if (cnt <= m.surv_cap) break;
     // Overflow: the counts are complete, the list is not. Grow and replay the batch
@@ -563,6 +667,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   }
   HIP_TRY(hipStreamSynchronize(m.stream));
   for (u32 w = 0; w < nwin; ++w) out.counts[w] = WinCounts{raw[w * 3], raw[w * 3 + 1], raw[w * 3 + 2]};
+  for (uint8_t &f : out.surv_prime) f = f != 0;   // pending after every base == prime
   return 0;
 }
 
@@ -591,24 +696,26 @@ int DeviceCtx::primality(const u64 *vals, size_t n, uint8_t *flags) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
   if (n == 0) return 0;
-  if (n > 0xFFFFFFFFull) { err_ = "too many values"; return -1; }
+  if (n > 0x0FFFFFFFull) { err_ = "too many values"; return -1; }
   HIP_TRY(hipSetDevice(device_));
-  u64 *d_v = nullptr;
+  u64 *d_v = nullptr, *d_one = nullptr;
   uint8_t *d_f = nullptr;
   u32 *d_n = nullptr;
   HIP_TRY(hipMalloc(&d_v, n * 8));
+  HIP_TRY(hipMalloc(&d_one, n * 8));
   HIP_TRY(hipMalloc(&d_f, n));
   HIP_TRY(hipMalloc(&d_n, 4));
   const u32 n32 = (u32)n;
   HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
-  dev::k_mr<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(d_v, d_n, n32, d_f);
+  dev::k_mr_base2<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(d_v, d_n, n32, d_f, d_one);
+  HIP_TRY(hipGetLastError());
+  dev::k_mr_bases<<<std::max(1, cus_ * 8), 256, 0, m.stream>>>(d_v, d_n, n32, d_f, d_one);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(flags, d_f, n, hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
-  HIP_TRY(hipFree(d_v));
-  HIP_TRY(hipFree(d_f));
-  HIP_TRY(hipFree(d_n));
+  for (size_t i = 0; i < n; ++i) flags[i] = flags[i] != 0;
+  for (void *p : {(void *)d_v, (void *)d_one, (void *)d_f, (void *)d_n}) HIP_TRY(hipFree(p));
   return 0;
 }
 

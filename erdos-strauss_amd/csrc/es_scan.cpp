@@ -10,6 +10,8 @@
 #include <functional>
 #include <map>
 #include <numeric>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 
 #include "es_tail.h"
@@ -30,22 +32,91 @@ u64 gcd64(u64 a, u64 b) {
 
 }  // namespace
 
+namespace {
+
+// Workers that live for the process: a parallel_for hands them an index range and
+// waits; spawning threads per call would cost more than the work it spreads.
+class Pool {
+ public:
+  explicit Pool(int n) {
+    for (int i = 0; i < n; ++i) workers_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread &t : workers_) t.join();
+  }
+  int size() const { return (int)workers_.size(); }
+  // Runs fn(i) for i < n on up to `width` workers plus the caller.
+  void run(size_t n, int width, const std::function<void(size_t)> &fn) {
+    std::unique_lock<std::mutex> call(call_mu_);   // one parallel_for at a time
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      n_ = n;
+      next_.store(0);
+      active_ = std::min(width, (int)workers_.size());
+      pending_ = active_;
+      ++gen_;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [this] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void drain() {
+    for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+  }
+  void loop() {
+    u64 seen = 0;
+    for (;;) {
+      int slot;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        if (active_ <= 0) continue;
+        slot = active_--;
+      }
+      (void)slot;
+      drain();
+      std::lock_guard<std::mutex> g(mu_);
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_, call_mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t)> *fn_ = nullptr;
+  size_t n_ = 0;
+  std::atomic<size_t> next_{0};
+  int active_ = 0, pending_ = 0;
+  u64 gen_ = 0;
+  bool stop_ = false;
+};
+
+Pool &pool() {
+  static Pool p((int)std::max(1u, std::thread::hardware_concurrency()) - 1);
+  return p;
+}
+
+}  // namespace
+
 void parallel_for(size_t n, int threads, const std::function<void(size_t)> &fn) {
   if (n == 0) return;
-  int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
-  T = (int)std::min<size_t>((size_t)T, n);
-  if (T <= 1) {
+  const int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  if (T <= 1 || n == 1) {
     for (size_t i = 0; i < n; ++i) fn(i);
     return;
   }
-  std::atomic<size_t> next{0};
-  std::vector<std::thread> pool;
-  pool.reserve(T);
-  for (int t = 0; t < T; ++t)
-    pool.emplace_back([&]() {
-      for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
-    });
-  for (std::thread &th : pool) th.join();
+  pool().run(n, (int)std::min<size_t>((size_t)T - 1, n - 1), fn);
 }
 
 void shard_interval(u64 &lo, u64 &hi, u64 shard, u64 nshard) {
@@ -99,6 +170,14 @@ int Scanner::open_device(int device, std::string &msg) {
 
 u64 Scanner::choose_layout(u64 w) const {
   if (tune_.fixed_Mp) return tune_.fixed_Mp;
+  if (!tune_.table.empty()) {
+    const double lw = std::log2((double)std::max<u64>(w, 1));
+    u64 best = tune_.table[0].second;
+    double gap = 1e300;
+    for (const auto &e : tune_.table)
+      if (std::fabs(e.first - lw) < gap) { gap = std::fabs(e.first - lw); best = e.second; }
+    return best;
+  }
   const double nf = (double)plan_.filters.mod.size();
   const double maxb = (double)tune_.launch.max_bits;
   u64 best = plan_.M;
@@ -111,6 +190,82 @@ u64 Scanner::choose_layout(u64 w) const {
     if (cost < best_cost) { best_cost = cost; best = e.Mp; }
   }
   return best;
+}
+
+int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, std::string &msg) {
+  if (!dev_open_) { msg = "FATAL: HIP device not open"; return -1; }
+  const Tuning saved = tune_;
+  char line[256];
+  // Reference height: the median window start of the sample (10^15 without one), kept
+  // clear of the ceiling so every probe window is valid.
+  std::vector<u64> los;
+  for (const Interval &w : sample) if (w.hi > w.lo) los.push_back(w.lo);
+  std::sort(los.begin(), los.end());
+  u64 base = los.empty() ? 1000000000000000ull : los[los.size() / 2];
+  const u64 floor_ = std::max(plan_.wheel.max_pmin, plan_.cover_pmax) + 1;
+  base = std::max(base, floor_);
+  if (base > kDirectPMax - (1ull << 40)) base = kDirectPMax - (1ull << 40);
+  auto time_batch = [&](const std::vector<Interval> &wins, double &sieve_ms) -> int {
+    std::vector<WindowRecord> recs;
+    BatchResult br;
+    ScanTiming tm;
+    if (device_pass(wins, recs, br, tm, msg)) return -1;      // warm (uploads the layout)
+    std::vector<double> t;
+    for (int r = 0; r < 3; ++r) {
+      if (device_pass(wins, recs, br, tm, msg)) return -1;
+      t.push_back(tm.sieve_ms > 0 ? tm.sieve_ms : tm.device_ms);
+    }
+    std::sort(t.begin(), t.end());
+    sieve_ms = t[1];
+    return 0;
+  };
+  tune_.table.clear();
+  tune_.launch.graph = false;
+  std::vector<std::pair<double, u64>> table;
+  for (int lw = 16; lw <= 34; lw += 2) {
+    const u64 W = 1ull << lw;
+    const u64 n = std::max<u64>(1, std::min<u64>(256, (1ull << 30) / W));
+    std::vector<Interval> wins;
+    for (u64 i = 0; i < n; ++i) wins.push_back(Interval{base + i * W, base + (i + 1) * W});
+    u64 best = 0;
+    double best_ms = 1e300;
+    for (const Estimate &e : est_) {
+      tune_.fixed_Mp = e.Mp;
+      double ms;
+      if (time_batch(wins, ms)) { tune_ = saved; return -1; }
+      snprintf(line, sizeof line, "layout W=2^%d x%llu Mp=%llu sieve_ms=%.4f\n", lw, (unsigned long long)n,
+               (unsigned long long)e.Mp, ms);
+      report += line;
+      if (ms < best_ms) { best_ms = ms; best = e.Mp; }
+    }
+    table.emplace_back((double)lw, best);
+  }
+  tune_.fixed_Mp = saved.fixed_Mp;
+  tune_.table = table;
+  // Launch shape on the sample batch itself.
+  if (!sample.empty()) {
+    LaunchCfg best_cfg = tune_.launch;
+    double best_ms = 1e300;
+    for (u32 block : {128u, 256u, 512u})
+      for (u32 bits : {8192u, 16384u, 32768u}) {
+        tune_.launch.block = block;
+        tune_.launch.max_bits = bits;
+        double ms;
+        if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+        snprintf(line, sizeof line, "launch block=%u tile_bits=%u sieve_ms=%.4f\n", block, bits, ms);
+        report += line;
+        if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
+      }
+    tune_.launch = best_cfg;
+  }
+  tune_.launch.graph = saved.launch.graph;
+  for (const auto &e : tune_.table) {
+    snprintf(line, sizeof line, "choice W=2^%.0f Mp=%llu\n", e.first, (unsigned long long)e.second);
+    report += line;
+  }
+  snprintf(line, sizeof line, "choice block=%u tile_bits=%u\n", tune_.launch.block, tune_.launch.max_bits);
+  report += line;
+  return 0;
 }
 
 int Scanner::mark_count(u64 lo, u64 hi, u64 Mp, u64 &marks, u64 &bits, u64 &pairs, u64 &tiles,

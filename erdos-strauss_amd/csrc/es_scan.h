@@ -37,10 +37,13 @@ enum class Order {
 };
 
 struct Tuning {
-  // Cost model in VALU-op units per window of W integers in layout Mp:
+  // Cost model in VALU-op units per window of W integers in layout Mp, used until the
+  // autotuner has measured the device:
   //   tiles * nf * c_pair  +  W * (marks_per_int * c_mark + bits_per_int * c_bit)
   double c_pair = 24.0, c_mark = 7.0, c_bit = 0.15;
   u64 fixed_Mp = 0;           // nonzero: always this layout (must divide M)
+  // Measured: the fastest layout per window size (log2 W -> Mp), from autotune().
+  std::vector<std::pair<double, u64>> table;
   LaunchCfg launch;
 };
 
@@ -68,6 +71,11 @@ class Scanner {
 
   // Layout the tuner picks for a window of w integers.
   u64 choose_layout(u64 w) const;
+  // Measures the device: kernel time of every candidate layout for window sizes 2^16
+  // .. 2^34 (at the sample's median height) -> the size-to-layout table; then workgroup
+  // size x tile size on the sample batch itself. Returns 0, or -1 (msg) on a device
+  // error; `report` gets one line per measurement.
+  int autotune(const std::vector<Interval> &sample, std::string &report, std::string &msg);
   // Makes sure the layout for Mp exists on the device; returns its device id or -1.
   int ensure_layout(u64 Mp, std::string &msg);
   // The host tables of layout Mp, built on first use (no device needed); null on error.

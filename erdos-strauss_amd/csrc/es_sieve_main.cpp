@@ -261,7 +261,7 @@ struct Cli {
   int threads = 0, device = 0;
   const char *filters = nullptr;
   bool emit_surv = true, plain = false, profile = false, quiet_time = false;
-  u32 block = 256, tile_bits = 65536;
+  u32 block = 256, tile_bits = 32768;
   u64 layout = 0, chunk = 1ull << 40;
   Order order = Order::Lane;
 };
@@ -337,8 +337,8 @@ int main(int argc, char **argv) {
   }
   if (!o.seg) o.seg = 262144;
   if (o.block == 0 || o.block > 1024 || o.block % 64 || o.tile_bits == 0 || o.tile_bits % 32 ||
-      o.tile_bits > (1u << 18)) {
-    fprintf(stderr, "FATAL: --block must be a multiple of 64 up to 1024, --tile-bits a multiple of 32 up to 2^18\n");
+      o.tile_bits > kMaxTileBits) {
+    fprintf(stderr, "FATAL: --block must be a multiple of 64 up to 1024, --tile-bits a multiple of 32 up to 61440\n");
     return 1;
   }
   if (o.chunk == 0) o.chunk = 1ull << 40;

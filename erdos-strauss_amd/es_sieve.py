@@ -72,7 +72,9 @@ def lib():
             "es_set_layout": (ctypes.c_int, [ctypes.c_void_p, U64]),
             "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),
             "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+            "es_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
             "es_choose_layout": (U64, [ctypes.c_void_p, U64]),
+            "es_autotune": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, P(ctypes.c_void_p)]),
             "es_last_timing": (ctypes.c_int, [ctypes.c_void_p, P(EsTiming)]),
             "es_wheel_info": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), P(U64), P(U64)]),
             "es_mark_count": (ctypes.c_int, [ctypes.c_void_p, U64, U64, U64, P(U64), P(U64), P(U64), P(U64)]),
@@ -188,11 +190,25 @@ class Scanner:
     def set_layout(self, Mp):
         _check(lib().es_set_layout(self.h, Mp))
 
-    def set_launch(self, persistent=True, block=256, max_bits=65536):
+    def set_launch(self, persistent=True, block=256, max_bits=32768):
         _check(lib().es_set_launch(self.h, 1 if persistent else 0, block, max_bits))
+
+    def set_graph(self, on=True):
+        _check(lib().es_set_graph(self.h, 1 if on else 0))
 
     def set_tuning(self, c_pair, c_mark, c_bit):
         _check(lib().es_set_tuning(self.h, c_pair, c_mark, c_bit))
+
+    def autotune(self, sample=()):
+        """Measures layouts per window size and the launch shape; returns the report."""
+        rep = ctypes.c_void_p()
+        n = len(sample)
+        _check(lib().es_autotune(self.h, _u64s([w[0] for w in sample]) if n else None,
+                                 _u64s([w[1] for w in sample]) if n else None, n, ctypes.byref(rep)))
+        try:
+            return ctypes.string_at(rep).decode()
+        finally:
+            lib().es_free(rep)
 
     def choose_layout(self, w):
         return lib().es_choose_layout(self.h, w)

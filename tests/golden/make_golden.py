@@ -157,8 +157,41 @@ def c5_trace(seconds=60.0):
             f.write("%.6f %d %d\n" % (t, start, start + length))
 
 
+ORACLE_BIN = os.path.join(REPO, "oracle", "es_oracle")
+
+
+def c4_records():
+    """Per-window records of the 512 batched windows for both wheels, from the oracle
+    (itself pinned to the reference binary and its golden records by tests/test_oracle.py);
+    windows with hi < 1e15, where the reference binary's startup is cheap, are also run
+    through the reference binary and must agree byte for byte."""
+    wins = [tuple(map(int, l.split())) for l in open(os.path.join(HERE, "c4_windows.txt"))
+            if not l.startswith("#")]
+    d = os.path.join(HERE, "c4")
+    os.makedirs(d, exist_ok=True)
+    for M, F in ((2042040, F204), (120120, F120)):
+        out = []
+        checked = 0
+        for lo, hi in wins:
+            argv = [str(lo), str(hi), "--wheel", str(M), "--filters", F]
+            rec = subprocess.run([ORACLE_BIN] + argv + ["--threads", "8"], stdout=subprocess.PIPE,
+                                 stderr=subprocess.DEVNULL).stdout
+            if hi < 10 ** 15:
+                ref = subprocess.run([REF_BIN] + argv, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL).stdout
+                if ref != rec:
+                    raise SystemExit("oracle and reference disagree on window %d %d" % (lo, hi))
+                checked += 1
+            out.append(rec)
+        with open(os.path.join(d, "records_w%d.txt" % M), "wb") as f:
+            f.write(b"".join(out))
+        print("c4 wheel", M, "records", len(out), "cross-checked against the reference:", checked, flush=True)
+
+
 def main():
     skip_slow = "--skip-slow" in sys.argv
+    if "--c4-only" in sys.argv:
+        c4_records()
+        return
     os.makedirs(os.path.join(HERE, "records"), exist_ok=True)
     c4_windows()
     c5_trace()
