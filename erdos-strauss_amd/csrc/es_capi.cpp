@@ -148,6 +148,34 @@ int es_certify(uint64_t p, char *buf, size_t cap) {
   return kind;
 }
 
+int es_certify_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, size_t *on_host) {
+  if (!s || !text || (n && !p)) return fail(ES_ERR_ARG, "null argument");
+  std::vector<es::RungResult> rr(n);
+  if (s->sc.device().rung(p, n, rr.data())) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  std::string all, line;
+  size_t host = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (rr[i].status == 1) {
+      es::format_rung(p[i], rr[i].A, rr[i].r, ((es::u128)rr[i].u_hi << 64) | rr[i].u_lo, line);
+    } else {
+      ++host;
+      es::certify_prime(p[i], line);
+    }
+    all += line;
+  }
+  *text = (char *)malloc(all.size() + 1);
+  if (!*text) return fail(ES_ERR_ARG, "out of host memory");
+  memcpy(*text, all.c_str(), all.size() + 1);
+  if (on_host) *on_host = host;
+  return ES_OK;
+}
+
+int es_set_device_tail(es_scanner *s, int on) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  s->sc.tuning().launch.device_tail = on != 0;
+  return ES_OK;
+}
+
 int es_set_layout(es_scanner *s, uint64_t Mp) {
   if (!s) return fail(ES_ERR_ARG, "null scanner");
   if (Mp && s->sc.plan().M % Mp) return fail(ES_ERR_ARG, "layout modulus must divide the wheel modulus");
@@ -206,7 +234,7 @@ int es_last_timing(const es_scanner *s, es_timing *t) {
   if (!s || !t) return fail(ES_ERR_ARG, "null argument");
   const es::ScanTiming &m = s->last;
   *t = es_timing{m.plan_ms, m.batch_ms, m.device_ms, m.sieve_ms, m.tail_ms, m.total_ms,
-                 m.tiles, m.survivors, m.primes, m.replays};
+                 m.tiles, m.survivors, m.primes, m.host_primes, m.replays};
   return ES_OK;
 }
 

@@ -29,11 +29,19 @@ struct WinCounts {
   u64 survivors;      // alive after stage B' (== after_covers without a sieve)
 };
 
+// Stage D computed on the device for one survivor. status: 1 = RUNG (r, A, u), 2 = no
+// rung up to 255 has a witness, 3 = the device declined (the host finishes), 0 = not prime.
+struct RungResult {
+  u64 A, u_lo, u_hi;
+  u32 r, status;
+};
+
 struct BatchResult {
   std::vector<WinCounts> counts;      // [nwin]
   std::vector<u64> surv_p;            // stage-C survivors (any order)
   std::vector<u32> surv_w;            // their window index
   std::vector<uint8_t> surv_prime;    // 1 if prime (deterministic MR on the device)
+  std::vector<RungResult> rung;       // per survivor, when stage D ran on the device
   double device_ms = 0;               // sieve + MR kernels, HIP events
   double sieve_ms = 0;                // sieve kernel alone
   int replays = 0;                    // survivor-buffer overflows that forced a rerun
@@ -48,6 +56,7 @@ struct LaunchCfg {
   bool graph = false;       // replay a captured hipGraph (persistent form only)
   u32 block = 256;          // threads per workgroup
   u32 max_bits = 32768;     // largest tile in positions (LDS bytes of the byte map)
+  bool device_tail = true;  // stage D on the device (k_rung); the host finishes what it declines
 };
 
 class DeviceCtx {
@@ -66,6 +75,8 @@ class DeviceCtx {
           BatchResult &out);
   // Deterministic MR of arbitrary values on the device (parity checks).
   int primality(const u64 *vals, size_t n, uint8_t *flags);
+  // Stage D on the device for primes vals[0..n) (one RungResult each).
+  int rung(const u64 *vals, size_t n, RungResult *out);
   // Measured LDS atomic-AND rate of the whole device (32-bit operations per second).
   int lds_peak(double &ops_per_s);
   const std::string &last_error() const { return err_; }

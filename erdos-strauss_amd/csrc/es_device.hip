@@ -346,6 +346,282 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
   }
 }
 
+// ------------------------------------------------------------------ stage D on the device
+//
+// One wave per prime (rung_scan3.cpp:574-603): rungs r == -p (mod 4) from the smallest,
+// skipping gcd(pA mod r, r) > 1; A = (p+r)/4 is factored completely (wave-parallel
+// trial division by the odd primes below 2^16, then Miller-Rabin, a square test and a
+// 64-walk Brent rho for what is left); the least witness u | (pA)^2 with u == -pA
+// (mod r) and u <= pA is a per-residue minimum carried prime power by prime power. The
+// certificate is a function of p alone, so computing it here or on the host gives the
+// same line; a prime the device declines is finished on the host.
+
+typedef unsigned __int128 u128;
+
+struct RungOut {
+  u64 A, u_lo, u_hi;
+  u32 r, status;
+};
+constexpr u32 kRungFound = 1, kRungNone = 2, kRungHost = 3;
+
+struct TrialTab {
+  const u32 *p;     // odd primes 3 .. 65521 ascending
+  const u64 *inv;   // p^{-1} mod 2^64
+  const u64 *lim;   // floor((2^64-1)/p)
+  u32 n;
+};
+
+// a * b <= q (a, b, q < 2^127), exactly, without a 128-bit division.
+__device__ __forceinline__ bool mul_le(u128 a, u128 b, u128 q, u128 &out) {
+  const u64 ah = (u64)(a >> 64), bh = (u64)(b >> 64);
+  if (ah && bh) return false;
+  const u128 big = ah ? a : b;
+  const u64 small = ah ? (u64)b : (u64)a;
+  const u128 t0 = (u128)(u64)big * small;
+  const u128 t1 = (u128)(u64)(big >> 64) * small;
+  if (t1 >> 64) return false;
+  const u128 sum = t0 + (t1 << 64);
+  if (sum < t0 || sum > q) return false;
+  out = sum;
+  return true;
+}
+
+__device__ __forceinline__ u64 gcd_u64(u64 a, u64 b) {
+  if (!a) return b;
+  if (!b) return a;
+  const int sh = __builtin_ctzll(a | b);
+  a >>= __builtin_ctzll(a);
+  do {
+    b >>= __builtin_ctzll(b);
+    if (a > b) { const u64 t = a; a = b; b = t; }
+    b -= a;
+  } while (b);
+  return a << sh;
+}
+
+__device__ __forceinline__ u64 neg_inv(u64 n) {
+  u64 x = n;
+  for (int k = 0; k < 6; ++k) x *= 2 - n * x;
+  return 0 - x;
+}
+
+// a * 2^64 mod n for a < 2^8, by double-and-add from one = 2^64 mod n.
+__device__ __forceinline__ u64 small_to_mont(u32 a, u64 one, u64 n) {
+  u64 am = 0;
+  for (int bit = 7; bit >= 0; --bit) {
+    am = add_mod(am, am, n);
+    if ((a >> bit) & 1) am = add_mod(am, one, n);
+  }
+  return am;
+}
+
+// Deterministic primality of odd n > 37 with the twelve prime bases tested on twelve
+// lanes at once; every lane gets the verdict.
+__device__ bool wave_is_prime(u64 n, u32 lane) {
+  const u64 ninv = neg_inv(n), one = (0 - n) % n;
+  u64 d = n - 1;
+  const u32 s = (u32)__builtin_ctzll(d);
+  d >>= s;
+  bool ok = true;
+  if (lane < 12) ok = strong_probable(n, ninv, one, c_small_p[lane], d, s);
+  return __ballot(!ok) == 0;
+}
+
+// Brent's rho with an independent walk per lane (x -> x^2 + c, c = lane + 1, Montgomery
+// form). Returns a nontrivial factor of the odd composite non-square n, or 0 when every
+// walk fails inside the budget (the caller then hands the prime to the host).
+__device__ u64 wave_rho(u64 n, u32 lane) {
+  const u64 ninv = neg_inv(n), one = (0 - n) % n;
+  const u64 c = small_to_mont(lane + 1, one, n);
+  auto f = [&](u64 v) { return add_mod(mont_mul(v, v, n, ninv), c, n); };
+  u64 y = small_to_mont(2, one, n), x = y, ys = y, acc = one;
+  bool alive = true;
+  for (u32 r = 1; r <= (1u << 16); r <<= 1) {
+    x = y;
+    for (u32 i = 0; i < r; ++i) y = f(y);
+    for (u32 k = 0; k < r; k += 64) {
+      ys = y;
+      const u32 lim = min(64u, r - k);
+      for (u32 i = 0; i < lim; ++i) {
+        y = f(y);
+        acc = mont_mul(acc, x > y ? x - y : y - x, n, ninv);
+      }
+      u64 g = alive ? gcd_u64(acc, n) : 1;
+      if (g == n) {                       // the batch overshot: replay it step by step
+        g = 1;
+        for (u32 i = 0; i < lim && g == 1; ++i) {
+          ys = f(ys);
+          const u64 dd = x > ys ? x - ys : ys - x;
+          g = gcd_u64(dd ? dd : n, n);
+        }
+        if (g == n) g = 1;
+        if (g == 1) alive = false;
+      }
+      const u64 found = __ballot(g != 1);
+      if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
+      if (__ballot(alive) == 0) return 0;
+    }
+  }
+  return 0;
+}
+
+// Per-wave factor list in LDS.
+struct WaveFac {
+  u64 p[16];
+  u32 e[16];
+  u32 k;
+  u32 bad;
+};
+
+__device__ __forceinline__ void fac_add(WaveFac &F, u64 q, u32 e, u32 lane) {
+  if (lane == 0) {
+    u32 i = 0;
+    while (i < F.k && F.p[i] != q) ++i;
+    if (i < F.k) F.e[i] += e;
+    else if (F.k < 16) { F.p[F.k] = q; F.e[F.k] = e; ++F.k; }
+    else F.bad = 1;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ u64 isqrt_dev(u64 m) {
+  u64 s = (u64)sqrt((double)m);
+  while (s && s * s > m) --s;
+  while ((s + 1) * (s + 1) <= m) ++s;
+  return s;
+}
+
+// Complete factorization of a (< 2^62) into F; false if the device declines.
+__device__ bool wave_factor(u64 a, const TrialTab &T, WaveFac &F, u32 lane) {
+  if (lane == 0) { F.k = 0; F.bad = 0; }
+  __syncthreads();
+  u64 n = a;
+  if (!(n & 1)) {
+    const u32 e = (u32)__builtin_ctzll(n);
+    n >>= e;
+    fac_add(F, 2, e, lane);
+  }
+  bool early = false;
+  for (u32 c0 = 0; c0 < T.n; c0 += 64) {
+    const u64 q0 = T.p[c0];
+    if (q0 * q0 > n) { early = true; break; }
+    const u32 i = c0 + lane;
+    u64 inv = 0, lim = 0;
+    u32 q = 0;
+    bool hit = false;
+    if (i < T.n) {
+      q = T.p[i];
+      inv = T.inv[i];
+      lim = T.lim[i];
+      hit = n * inv <= lim;
+    }
+    u64 mask = __ballot(hit);
+    while (mask) {
+      const int l = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const u64 qi = __shfl(inv, l, 64), ql = __shfl(lim, l, 64);
+      const u32 qq = __shfl(q, l, 64);
+      u32 e = 0;
+      do { n *= qi; ++e; } while (n * qi <= ql);   // exact division by qq
+      fac_add(F, qq, e, lane);
+    }
+  }
+  if (n > 1) {
+    if (early || n <= 0xFFFFFFFFull) {
+      fac_add(F, n, 1, lane);
+    } else {
+      // Every prime factor of n exceeds 65521, so a part below 2^32 is prime.
+      u64 stack[4];
+      u32 top = 0;
+      stack[top++] = n;
+      while (top) {
+        const u64 m = stack[--top];
+        if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
+        const u64 s = isqrt_dev(m);
+        if (s * s == m) { fac_add(F, s, 2, lane); continue; }
+        const u64 d = wave_rho(m, lane);
+        if (!d || top + 2 > 4) return false;
+        stack[top++] = d;
+        stack[top++] = m / d;
+      }
+    }
+  }
+  return F.bad == 0;
+}
+
+__device__ __forceinline__ u32 small_mod(u64 x, u32 r, u32 mg, u32 c32) { return mod64(x, r, mg, c32); }
+
+__global__ __launch_bounds__(64) void k_rung(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
+                                             const u8 *__restrict__ flags, TrialTab T, RungOut *__restrict__ out) {
+  __shared__ u128 best[256], nxt[256];
+  __shared__ WaveFac F;
+  const u32 lane = threadIdx.x;
+  const u32 n_ = min(*count, cap);
+  const u128 INF = ~(u128)0;
+  for (u32 idx = blockIdx.x; idx < n_; idx += gridDim.x) {
+    if (!flags[idx]) continue;
+    const u64 p = vals[idx];
+    RungOut res{0, 0, 0, 0, kRungNone};
+    u32 r = (u32)((4 - (p & 3)) & 3);
+    if (r == 0) r = 4;
+    for (; r <= 255; r += 4) {
+      const u64 A = (p + r) / 4;
+      const u32 mg = r == 1 ? 0xFFFFFFFFu : (u32)(0x100000000ull / r), c32 = (u32)(0x100000000ull % r);
+      const u32 qm = (small_mod(p, r, mg, c32) * small_mod(A, r, mg, c32)) % r;
+      u32 ga = qm, gb = r;
+      while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
+      if (r > 1 && ga != 1) continue;
+      const u32 target = qm ? r - qm : 0;
+      if (!wave_factor(A, T, F, lane)) { res.status = kRungHost; break; }
+      const u128 q = (u128)p * A;
+      for (u32 t = lane; t < r; t += 64) best[t] = INF;
+      __syncthreads();
+      if (lane == 0) best[1 % r] = 1;
+      __syncthreads();
+      // Absorb p^2, then every prime power of A^2.
+      for (u32 fi = 0; fi <= F.k; ++fi) {
+        const u64 pr = fi == 0 ? p : F.p[fi - 1];
+        const u32 E = fi == 0 ? 2u : 2u * F.e[fi - 1];
+        const u32 pm = small_mod(pr, r, mg, c32);
+        // pm is a unit mod r on every rung that is not skipped: find its inverse.
+        u32 pinv = 0;
+        for (u32 x = lane; x < r; x += 64) if ((pm * x) % r == 1 % r) pinv = x + 1;
+        const u64 bal = __ballot(pinv != 0);
+        pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
+        for (u32 t = lane; t < r; t += 64) {
+          u128 acc = INF, pw = 1;
+          u32 s = t;
+          for (u32 j = 0;; ++j) {
+            const u128 b = best[s];
+            u128 v;
+            if (b != INF && mul_le(b, pw, q, v) && v < acc) acc = v;
+            if (j == E) break;
+            u128 npw;
+            if (!mul_le(pw, (u128)pr, q, npw)) break;
+            pw = npw;
+            s = (s * pinv) % r;
+          }
+          nxt[t] = acc;
+        }
+        __syncthreads();
+        for (u32 t = lane; t < r; t += 64) best[t] = nxt[t];
+        __syncthreads();
+      }
+      const u128 u = best[target];
+      __syncthreads();
+      if (u != INF) {
+        res.A = A;
+        res.u_lo = (u64)u;
+        res.u_hi = (u64)(u >> 64);
+        res.r = r;
+        res.status = kRungFound;
+        break;
+      }
+    }
+    if (lane == 0) out[idx] = res;
+  }
+}
+
 // LDS atomic-AND throughput probe: every thread clears bits of its own word for `iters`
 // rounds (consecutive threads on consecutive banks, so no bank conflicts); the word is
 // written out at the end so the work is observable.
@@ -389,6 +665,7 @@ struct DeviceCtx::Impl {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_ok = false;
   u32 graph_block = 0, graph_bits = 0;
+  bool graph_tail = false;
   std::vector<DevLayoutBufs> layouts;
   dev::LayoutDev *d_lays = nullptr;
   size_t lays_cap = 0;
@@ -402,6 +679,8 @@ struct DeviceCtx::Impl {
   u32 *d_surv_w = nullptr;
   uint8_t *d_flags = nullptr;
   u64 *d_one = nullptr;       // 2^64 mod n per survivor, from the base-2 pass
+  dev::RungOut *d_rung = nullptr;   // stage D result per survivor
+  dev::TrialTab trial{};            // odd primes below 2^16 with their inverses
   size_t surv_cap = 0;
   u32 *d_scalars = nullptr;   // [0] survivor count, [1] queue head
   u32 *h_count = nullptr;
@@ -421,7 +700,8 @@ DeviceCtx::~DeviceCtx() {
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
     for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_counts,
                     (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one,
-                    (void *)m.d_scalars})
+                    (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.p, (void *)m.trial.inv,
+                    (void *)m.trial.lim})
       if (p) (void)hipFree(p);
     if (m.h_hdr) (void)hipHostFree(m.h_hdr);
     if (m.h_count) (void)hipHostFree(m.h_count);
@@ -458,6 +738,28 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
   HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32)));
+  // Trial-division table for stage D: odd primes below 2^16, q^{-1} mod 2^64, floor((2^64-1)/q).
+  std::vector<uint8_t> comp(65537, 0);
+  std::vector<u32> tp;
+  std::vector<u64> tinv, tlim;
+  for (u32 i = 3; i <= 65536; i += 2) {
+    if (comp[i]) continue;
+    tp.push_back(i);
+    u64 x = i;
+    for (int k = 0; k < 6; ++k) x *= 2 - (u64)i * x;
+    tinv.push_back(x);
+    tlim.push_back(~0ull / i);
+    for (u32 j = i * i; j <= 65536; j += 2 * i) comp[j] = 1;
+  }
+  u32 *dp = nullptr;
+  u64 *dinv = nullptr, *dlim = nullptr;
+  HIP_TRY(hipMalloc(&dp, tp.size() * 4));
+  HIP_TRY(hipMalloc(&dinv, tinv.size() * 8));
+  HIP_TRY(hipMalloc(&dlim, tlim.size() * 8));
+  HIP_TRY(hipMemcpy(dp, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dinv, tinv.data(), tinv.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dlim, tlim.data(), tlim.size() * 8, hipMemcpyHostToDevice));
+  m.trial = dev::TrialTab{dp, dinv, dlim, (u32)tp.size()};
   return 0;
 }
 
@@ -521,6 +823,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   out.surv_p.clear();
   out.surv_w.clear();
   out.surv_prime.clear();
+  out.rung.clear();
   out.replays = 0;
   out.device_ms = out.sieve_ms = 0;
   if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
@@ -556,10 +859,11 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     size_t cap = std::max<size_t>(m.surv_cap ? m.surv_cap : (1u << 16), 1u << 16);
     while (cap < need) cap *= 2;
     if (cap > (1ull << 28)) { err_ = "survivor count exceeds 2^28 per batch"; return -1; }
-    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one})
+    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one, (void *)m.d_rung})
       if (p) HIP_TRY(hipFree(p));
     HIP_TRY(hipMalloc(&m.d_surv_p, cap * sizeof(u64)));
     HIP_TRY(hipMalloc(&m.d_one, cap * sizeof(u64)));
+    HIP_TRY(hipMalloc(&m.d_rung, cap * sizeof(dev::RungOut)));
     HIP_TRY(hipMalloc(&m.d_surv_w, cap * sizeof(u32)));
     HIP_TRY(hipMalloc(&m.d_flags, cap));
     m.surv_cap = cap;
@@ -614,13 +918,18 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
                                                                   m.d_flags, m.d_one);
     HIP_TRY(hipGetLastError());
+    if (cfg.device_tail) {
+      dev::k_rung<<<std::max(1, cus_ * 8), 64, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+                                                               m.d_flags, m.trial, m.d_rung);
+      HIP_TRY(hipGetLastError());
+    }
     return 0;
   };
   const bool use_graph = cfg.graph && cfg.persistent;
   for (;;) {
     HIP_TRY(hipEventRecord(m.ev0, m.stream));
     if (use_graph) {
-      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds_bytes) {
+      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds_bytes || m.graph_tail != cfg.device_tail) {
         if (m.graph_exec) { HIP_TRY(hipGraphExecDestroy(m.graph_exec)); m.graph_exec = nullptr; }
         if (m.graph) { HIP_TRY(hipGraphDestroy(m.graph)); m.graph = nullptr; }
         HIP_TRY(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
@@ -634,6 +943,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
         m.graph_ok = true;
         m.graph_block = cfg.block;
         m.graph_bits = lds_bytes;
+        m.graph_tail = cfg.device_tail;
       }
       HIP_TRY(hipGraphLaunch(m.graph_exec, m.stream));
     } else {
@@ -664,10 +974,43 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     HIP_TRY(hipMemcpyAsync(out.surv_p.data(), m.d_surv_p, cnt * sizeof(u64), hipMemcpyDeviceToHost, m.stream));
     HIP_TRY(hipMemcpyAsync(out.surv_w.data(), m.d_surv_w, cnt * sizeof(u32), hipMemcpyDeviceToHost, m.stream));
     HIP_TRY(hipMemcpyAsync(out.surv_prime.data(), m.d_flags, cnt, hipMemcpyDeviceToHost, m.stream));
+    if (cfg.device_tail) {
+      static_assert(sizeof(dev::RungOut) == sizeof(RungResult), "RungOut mirrors RungResult");
+      out.rung.resize(cnt);
+      HIP_TRY(hipMemcpyAsync(out.rung.data(), m.d_rung, cnt * sizeof(RungResult), hipMemcpyDeviceToHost,
+                             m.stream));
+    }
   }
   HIP_TRY(hipStreamSynchronize(m.stream));
   for (u32 w = 0; w < nwin; ++w) out.counts[w] = WinCounts{raw[w * 3], raw[w * 3 + 1], raw[w * 3 + 2]};
   for (uint8_t &f : out.surv_prime) f = f != 0;   // pending after every base == prime
+  return 0;
+}
+
+int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  if (n == 0) return 0;
+  if (n > 0x0FFFFFFFull) { err_ = "too many values"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  u64 *d_v = nullptr;
+  uint8_t *d_f = nullptr;
+  u32 *d_n = nullptr;
+  dev::RungOut *d_o = nullptr;
+  HIP_TRY(hipMalloc(&d_v, n * 8));
+  HIP_TRY(hipMalloc(&d_f, n));
+  HIP_TRY(hipMalloc(&d_n, 4));
+  HIP_TRY(hipMalloc(&d_o, n * sizeof(dev::RungOut)));
+  const u32 n32 = (u32)n;
+  std::vector<uint8_t> ones(n, 1);
+  HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemcpyAsync(d_f, ones.data(), n, hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
+  dev::k_rung<<<std::max(1, cus_ * 8), 64, 0, m.stream>>>(d_v, d_n, n32, d_f, m.trial, d_o);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
+  HIP_TRY(hipStreamSynchronize(m.stream));
+  for (void *p : {(void *)d_v, (void *)d_f, (void *)d_n, (void *)d_o}) HIP_TRY(hipFree(p));
   return 0;
 }
 

@@ -419,10 +419,24 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
     if (br.surv_prime[i]) prime_idx.push_back(i);
   tm.primes = prime_idx.size();
 
-  // Host tail: stage D then E per prime, in parallel; lines land in per-prime slots.
+  // Stage D results from the device become lines here; what the device declined (and
+  // stage E) runs on the host, in parallel, into per-prime slots.
   std::vector<std::string> lines(prime_idx.size());
-  std::vector<int> kinds(prime_idx.size());
-  parallel_for(prime_idx.size(), threads, [&](size_t j) {
+  std::vector<int> kinds(prime_idx.size(), -1);
+  std::vector<size_t> host_j;
+  for (size_t j = 0; j < prime_idx.size(); ++j) {
+    const size_t i = prime_idx[j];
+    if (!br.rung.empty() && br.rung[i].status == 1) {
+      const RungResult &g = br.rung[i];
+      format_rung(br.surv_p[i], g.A, g.r, ((u128)g.u_hi << 64) | g.u_lo, lines[j]);
+      kinds[j] = 0;
+    } else {
+      host_j.push_back(j);
+    }
+  }
+  tm.host_primes = host_j.size();
+  parallel_for(host_j.size(), threads, [&](size_t h) {
+    const size_t j = host_j[h];
     kinds[j] = certify_prime(br.surv_p[prime_idx[j]], lines[j]);
   });
   const double t3 = now_ms();

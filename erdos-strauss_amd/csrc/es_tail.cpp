@@ -282,11 +282,7 @@ bool solve_rung(u64 p, std::string &line) {
     }
     const u128 u = least_witness(p, fa, q, r, target);
     if (!u) continue;
-    char head[96];
-    snprintf(head, sizeof head, "RUNG %llu %llu %u ", (unsigned long long)p, (unsigned long long)A, r);
-    line = head;
-    append_u128(line, u);
-    line.push_back('\n');
+    format_rung(p, A, r, u, line);
     return true;
   }
   return false;
@@ -329,6 +325,14 @@ bool solve_direct(u64 p, std::string &line) {
       }
     }
   return false;
+}
+
+void format_rung(u64 p, u64 A, unsigned r, u128 u, std::string &line) {
+  char head[96];
+  snprintf(head, sizeof head, "RUNG %llu %llu %u ", (unsigned long long)p, (unsigned long long)A, r);
+  line = head;
+  append_u128(line, u);
+  line.push_back('\n');
 }
 
 int certify_prime(u64 p, std::string &line) {

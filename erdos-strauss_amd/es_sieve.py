@@ -34,7 +34,8 @@ class EsCounts(ctypes.Structure):
 class EsTiming(ctypes.Structure):
     _fields_ = [("plan_ms", ctypes.c_double), ("batch_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
                 ("sieve_ms", ctypes.c_double), ("tail_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("tiles", U64), ("survivors", U64), ("primes", U64), ("replays", ctypes.c_int)]
+                ("tiles", U64), ("survivors", U64), ("primes", U64), ("host_primes", U64),
+                ("replays", ctypes.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -73,6 +74,9 @@ def lib():
             "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),
             "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "es_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+            "es_set_device_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+            "es_certify_batch": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_void_p),
+                                                P(ctypes.c_size_t)]),
             "es_choose_layout": (U64, [ctypes.c_void_p, U64]),
             "es_autotune": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, P(ctypes.c_void_p)]),
             "es_last_timing": (ctypes.c_int, [ctypes.c_void_p, P(EsTiming)]),
@@ -195,6 +199,20 @@ class Scanner:
 
     def set_graph(self, on=True):
         _check(lib().es_set_graph(self.h, 1 if on else 0))
+
+    def set_device_tail(self, on=True):
+        _check(lib().es_set_device_tail(self.h, 1 if on else 0))
+
+    def certify_batch(self, primes):
+        """Stage D on the device for a list of primes: (lines, how many the host finished)."""
+        text = ctypes.c_void_p()
+        host = ctypes.c_size_t()
+        _check(lib().es_certify_batch(self.h, _u64s(primes), len(primes), ctypes.byref(text), ctypes.byref(host)))
+        try:
+            out = ctypes.string_at(text).decode()
+        finally:
+            lib().es_free(text)
+        return out.splitlines(keepends=True), host.value
 
     def set_tuning(self, c_pair, c_mark, c_bit):
         _check(lib().es_set_tuning(self.h, c_pair, c_mark, c_bit))

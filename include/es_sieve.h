@@ -50,7 +50,7 @@ typedef struct es_counts {
 /* Timing of the last scan call, milliseconds (scan.cu:150-162's --profile split). */
 typedef struct es_timing {
   double plan_ms, batch_ms, device_ms, sieve_ms, tail_ms, total_ms;
-  uint64_t tiles, survivors, primes;
+  uint64_t tiles, survivors, primes, host_primes;
   int replays;
 } es_timing;
 
@@ -102,6 +102,16 @@ int es_primality(es_scanner *s, const uint64_t *vals, size_t n, uint8_t *flags);
  * buf is too small). Replaces ref3_solve_rung / ref3_solve_direct
  * (sieve/cuda/ref_host.h:20-21; rung_scan3.cpp:578-647). */
 int es_certify(uint64_t p, char *buf, size_t cap);
+
+/* Stage D on the device for n primes: one wave per prime (k_rung), lines formatted on
+ * the host exactly as es_certify writes them; a prime the device declines is finished
+ * by the host. Lines are concatenated into *text (free with es_free); *on_host
+ * (optional) receives how many primes the host finished. The parity hook of the device
+ * tail (the reference keeps stage D on the host, sieve/cuda/ref_host.h:20-21). */
+int es_certify_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, size_t *on_host);
+
+/* Stage D placement for the following scans: 1 = on the device (default), 0 = host. */
+int es_set_device_tail(es_scanner *s, int on);
 
 /* Sub-wheel layout for the following scans: Mp must divide the wheel modulus; 0
  * restores the autotuner. No counterpart in the reference (whose only layout is

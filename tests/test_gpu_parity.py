@@ -232,6 +232,41 @@ def test_batch_equals_alone():
         assert rec == want
 
 
+# ---------------------------------------------------------------- stage D on the device
+
+GOLDEN_LINE_FILES = ["w19_certificates.txt", "w64_head.txt", "w64_tail.txt", "s0_head.txt", "s0_tail.txt",
+                     "s69_head.txt", "s69_tail.txt"]
+
+
+@pytest.mark.parametrize("fn", GOLDEN_LINE_FILES)
+def test_device_stage_d_reproduces_golden_lines(fn):
+    """k_rung (one wave per prime) re-derives every certificate line the reference
+    published, byte for byte: smallest rung, then smallest witness."""
+    want = [l for l in open(os.path.join(paths.GOLDEN, "rederive", fn)) if l.startswith("RUNG")]
+    primes = [int(l.split()[1]) for l in want]
+    sc = scanner(filters=F204)
+    got, on_host = sc.certify_batch(primes)
+    print(fn, len(primes), "primes,", on_host, "finished on the host", flush=True)
+    assert got == want
+    assert on_host <= len(primes) // 10
+
+
+def test_device_tail_on_and_off_agree():
+    """The same records whether stage D runs on the device or the host."""
+    kw = dict(wheel=120120, mmax=60, sieve=0, filters=None)
+    lo, hi = 10 ** 9, 10 ** 9 + 10 ** 8
+    sc = scanner(**kw)
+    want = paths.golden_record("mmax60_w120").decode()
+    try:
+        for tail in (True, False):
+            sc.set_device_tail(tail)
+            (rec,), _ = sc.records([(lo, hi)])
+            assert rec == want, tail
+            print("tail", tail, sc.timing(), flush=True)
+    finally:
+        sc.set_device_tail(True)
+
+
 # ---------------------------------------------------------------- Miller-Rabin
 
 def test_primality_dense_and_kats():
