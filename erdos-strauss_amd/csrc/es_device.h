@@ -91,6 +91,7 @@ class DeviceCtx {
   std::string err_;
   int device_ = -1;
   int cus_ = 0;
+  size_t lds_limit_ = 0;   // dynamic LDS one workgroup may allocate (bytes)
   size_t layout_bytes_ = 0;
 };
 

@@ -40,8 +40,8 @@ struct LayoutDev {
   const u32 *magic;
   const u32 *c32;
   const uint4 *fc;           // per ordered filter: {s, floor(2^32/s), 2^32 mod s, 0}
-  const u16 *off;
-  u32 P, Pmagic, Pc32, nl, nf, ncov, cov_small, pr_small, mask_words, pad;
+  const u16 *off;            // per lane a row of ostride offsets (nf used, 0xFFFF after)
+  u32 P, Pmagic, Pc32, nl, nf, ncov, cov_small, pr_small, mask_words, ostride;
 };
 
 struct BatchHdr {
@@ -114,7 +114,11 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
 // The tile is a byte map in LDS: byte k says whether position k0 + k is still alive.
 // A mark is a plain byte store of 0 -- idempotent, so concurrent stores need no atomics.
 // All LDS is one dynamic allocation: the map at offset 0 (so a mark's address is its
-// position), then a few scalars. lds_words counts the map in 32-bit words.
+// position), then the tile's row of filter offsets, then a few scalars. lds_words and
+// off_words count the map and the offset row in 32-bit words. The offset rows (one per
+// lane, megabytes in all) do not stay in L2, so each tile streams its row into LDS with
+// coalesced 16-byte loads that overlap the mask init, instead of every bin waiting on
+// its own global load.
 template <bool kPersistent>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
@@ -122,11 +126,12 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
                                                 u64 *__restrict__ counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
                                                 u32 *__restrict__ surv_count, u32 *__restrict__ queue,
-                                                u32 lds_words) {
+                                                u32 lds_words, u32 off_words) {
   extern __shared__ u32 lds[];
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
-  u32 *sc = lds + lds_words;   // [0] tile, [1] base, [2] bin counter B, [3] bin counter B',
+  u16 *soff = (u16 *)(lds + lds_words);
+  u32 *sc = lds + lds_words + off_words;   // [0] tile, [1] base, [2] bin counter B, [3] bin counter B',
                                // [16..) per-wave partial sums (3 x kMaxWaves)
   u32 *s_wpos = sc + 16, *s_waft = sc + 16 + kMaxWaves, *s_wsur = sc + 16 + 2 * kMaxWaves;
   const u32 tid = threadIdx.x, bs = blockDim.x;
@@ -144,9 +149,14 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     const TileDesc td = tiles[t];
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
-    const u16 *offrow = L->off + (size_t)td.lane * L->nf;
     const u32 P = L->P;
     if (tid == 0) { sc[2] = 0; sc[3] = 0; }
+    {
+      const uint4 *src = (const uint4 *)(L->off + (size_t)td.lane * L->ostride);
+      uint4 *dst = (uint4 *)soff;
+      for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
+    }
+    const u16 *offrow = soff;
 
     // ---- init: wheel mask and tail mask, expanded to one byte per position
     u32 my_pos = 0;
@@ -252,6 +262,23 @@ __device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 n) {
   return (s < a || s >= n) ? s - n : s;
 }
 
+// x * y mod n for x, y < n < 2^50, exact, in FP64 (full-rate FMAs instead of the
+// quarter-rate 32-bit multiplies of 64-bit Montgomery). h + l == x*y exactly (FMA
+// residual); q is floor(x*y/n) up to +-1, so h - q*n is an integer below 2^52 in
+// magnitude, computed exactly by one FMA; one correction each way finishes.
+__device__ __forceinline__ u64 mulmod_f64(u64 x, u64 y, double nd, double ninv) {
+  const double a = (double)x, b = (double)y;
+  const double h = a * b;
+  const double l = __fma_rn(a, b, -h);
+  const double q = floor(h * ninv);
+  double r = __fma_rn(-q, nd, h) + l;
+  if (r < 0) r += nd;
+  if (r >= nd) r -= nd;
+  return (u64)r;
+}
+
+constexpr u64 kF64Mod = 1ull << 50;
+
 // n odd > 37; d odd with n - 1 = d 2^s; base a small.
 __device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
   u64 am = 0;                                 // a * 2^64 mod n by double-and-add
@@ -269,6 +296,22 @@ __device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
   for (u32 i = 1; i < s; ++i) {
     x = mont_mul(x, x, n, ninv);
     if (x == minus_one) return true;
+  }
+  return false;
+}
+
+// The same test in plain residues with mulmod_f64, for n < 2^50.
+__device__ bool strong_probable_f64(u64 n, u64 a, u64 d, u32 s) {
+  const double nd = (double)n, ninv = 1.0 / nd;
+  u64 x = 1, b = a % n;
+  for (u64 e = d; e; e >>= 1) {
+    if (e & 1) x = mulmod_f64(x, b, nd, ninv);
+    b = mulmod_f64(b, b, nd, ninv);
+  }
+  if (x == 1 || x == n - 1) return true;
+  for (u32 i = 1; i < s; ++i) {
+    x = mulmod_f64(x, x, nd, ninv);
+    if (x == n - 1) return true;
   }
   return false;
 }
@@ -309,7 +352,12 @@ __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__
       bool divisible = false;
       for (int k = 0; k < 11; ++k) divisible |= n * c_div[k].inv <= c_div[k].lim;
       if (divisible) v = kComposite;
-      else {
+      else if (n < kF64Mod) {
+        u64 d = n - 1;
+        const u32 s = (u32)__builtin_ctzll(d);
+        d >>= s;
+        v = strong_probable_f64(n, 2, d, s) ? kPending : kComposite;
+      } else {
         u64 ninv = n;                         // n^{-1} mod 2^64 (Newton), then negate
         for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
         ninv = 0 - ninv;
@@ -336,25 +384,37 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
     const u32 i = g / 11u, b = g - i * 11u + 1u;
     if (flags[i] != kPending) continue;
     const u64 n = vals[i];
-    u64 ninv = n;
-    for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
-    ninv = 0 - ninv;
     u64 d = n - 1;
     const u32 s = (u32)__builtin_ctzll(d);
     d >>= s;
-    if (!strong_probable(n, ninv, one_in[i], c_small_p[b], d, s)) flags[i] = kComposite;
+    bool ok;
+    if (n < kF64Mod) {
+      ok = strong_probable_f64(n, c_small_p[b], d, s);
+    } else {
+      u64 ninv = n;
+      for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
+      ninv = 0 - ninv;
+      ok = strong_probable(n, ninv, one_in[i], c_small_p[b], d, s);
+    }
+    if (!ok) flags[i] = kComposite;
   }
 }
 
 // ------------------------------------------------------------------ stage D on the device
 //
-// One wave per prime (rung_scan3.cpp:574-603): rungs r == -p (mod 4) from the smallest,
-// skipping gcd(pA mod r, r) > 1; A = (p+r)/4 is factored completely (wave-parallel
-// trial division by the odd primes below 2^16, then Miller-Rabin, a square test and a
-// 64-walk Brent rho for what is left); the least witness u | (pA)^2 with u == -pA
-// (mod r) and u <= pA is a per-residue minimum carried prime power by prime power. The
-// certificate is a function of p alone, so computing it here or on the host gives the
-// same line; a prime the device declines is finished on the host.
+// One workgroup of kRungWaves waves per prime (rung_scan3.cpp:574-603). The candidate
+// rungs r = r0, r0 + 4, ... <= 255 (r0 == -p mod 4) have A_j = (p + r0)/4 + j, a run of
+// consecutive integers, so one workgroup-wide pass over the odd primes below 2^16
+// computes A_0 mod q once per prime and records every q | A_j for all 64 rungs at once
+// (a sieve instead of per-rung trial division). Waves then take rungs in order from an
+// LDS counter: skip gcd(pA mod r, r) > 1, finish A_j's factorization (exponents of the
+// sieved primes by exact division, then Miller-Rabin, a square test and a 64-walk Brent
+// rho for the cofactor, whose prime factors all exceed 65521), and search the least
+// witness u | (pA)^2 with u == -pA (mod r), u <= pA as a per-residue minimum carried
+// prime power by prime power. The least rung with a witness wins; a wave stops as soon
+// as a smaller rung is settled. The certificate is a function of p alone, so computing
+// it here or on the host gives the same line; a prime the device declines is finished
+// on the host.
 
 typedef unsigned __int128 u128;
 
@@ -363,13 +423,24 @@ struct RungOut {
   u32 r, status;
 };
 constexpr u32 kRungFound = 1, kRungNone = 2, kRungHost = 3;
+constexpr u32 kRungWaves = 4;     // waves per prime = rungs in flight
+constexpr u32 kRungSlots = 64;    // candidate rungs r0 + 4j <= 255
+constexpr u32 kSmallFac = 16;     // odd primes < 2^16 dividing one A < 2^62: at most 15
 
 struct TrialTab {
-  const u32 *p;     // odd primes 3 .. 65521 ascending
-  const u64 *inv;   // p^{-1} mod 2^64
-  const u64 *lim;   // floor((2^64-1)/p)
+  const uint4 *qm;  // odd primes 3 .. 65521 ascending: {q, floor(2^32/q), 2^32 mod q, 0}
+  const u64 *inv;   // q^{-1} mod 2^64
+  const u64 *lim;   // floor((2^64-1)/q)
   u32 n;
 };
+
+// Lane-to-lane hand-off through LDS inside one wave: a wave's LDS operations complete in
+// issue order, so this only has to stop the compiler from moving accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // a * b <= q (a, b, q < 2^127), exactly, without a 128-bit division.
 __device__ __forceinline__ bool mul_le(u128 a, u128 b, u128 q, u128 &out) {
@@ -399,11 +470,14 @@ __device__ __forceinline__ u64 gcd_u64(u64 a, u64 b) {
   return a << sh;
 }
 
-__device__ __forceinline__ u64 neg_inv(u64 n) {
+// n^{-1} mod 2^64 for odd n (Newton: each step doubles the correct low bits).
+__device__ __forceinline__ u64 inv64(u64 n) {
   u64 x = n;
   for (int k = 0; k < 6; ++k) x *= 2 - n * x;
-  return 0 - x;
+  return x;
 }
+
+__device__ __forceinline__ u64 neg_inv(u64 n) { return 0 - inv64(n); }
 
 // a * 2^64 mod n for a < 2^8, by double-and-add from one = 2^64 mod n.
 __device__ __forceinline__ u64 small_to_mont(u32 a, u64 one, u64 n) {
@@ -418,19 +492,28 @@ __device__ __forceinline__ u64 small_to_mont(u32 a, u64 one, u64 n) {
 // Deterministic primality of odd n > 37 with the twelve prime bases tested on twelve
 // lanes at once; every lane gets the verdict.
 __device__ bool wave_is_prime(u64 n, u32 lane) {
-  const u64 ninv = neg_inv(n), one = (0 - n) % n;
   u64 d = n - 1;
   const u32 s = (u32)__builtin_ctzll(d);
   d >>= s;
   bool ok = true;
-  if (lane < 12) ok = strong_probable(n, ninv, one, c_small_p[lane], d, s);
+  if (lane < 12) {
+    if (n < kF64Mod) {
+      ok = strong_probable_f64(n, c_small_p[lane], d, s);
+    } else {
+      const u64 ninv = neg_inv(n), one = (0 - n) % n;
+      ok = strong_probable(n, ninv, one, c_small_p[lane], d, s);
+    }
+  }
   return __ballot(!ok) == 0;
 }
 
+constexpr u64 kRhoFail = 0, kRhoStop = 1;
+
 // Brent's rho with an independent walk per lane (x -> x^2 + c, c = lane + 1, Montgomery
-// form). Returns a nontrivial factor of the odd composite non-square n, or 0 when every
-// walk fails inside the budget (the caller then hands the prime to the host).
-__device__ u64 wave_rho(u64 n, u32 lane) {
+// form). Returns a nontrivial factor of the odd composite non-square n, kRhoFail when
+// every walk fails inside the budget (the caller then hands the prime to the host), or
+// kRhoStop once *stop drops below k (a smaller rung is settled; the result is moot).
+__device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
   const u64 ninv = neg_inv(n), one = (0 - n) % n;
   const u64 c = small_to_mont(lane + 1, one, n);
   auto f = [&](u64 v) { return add_mod(mont_mul(v, v, n, ninv), c, n); };
@@ -439,9 +522,9 @@ __device__ u64 wave_rho(u64 n, u32 lane) {
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
     x = y;
     for (u32 i = 0; i < r; ++i) y = f(y);
-    for (u32 k = 0; k < r; k += 64) {
+    for (u32 kk = 0; kk < r; kk += 64) {
       ys = y;
-      const u32 lim = min(64u, r - k);
+      const u32 lim = min(64u, r - kk);
       for (u32 i = 0; i < lim; ++i) {
         y = f(y);
         acc = mont_mul(acc, x > y ? x - y : y - x, n, ninv);
@@ -459,10 +542,49 @@ __device__ u64 wave_rho(u64 n, u32 lane) {
       }
       const u64 found = __ballot(g != 1);
       if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
-      if (__ballot(alive) == 0) return 0;
+      if (__ballot(alive) == 0) return kRhoFail;
+      if (*stop < k) return kRhoStop;
     }
   }
-  return 0;
+  return kRhoFail;
+}
+
+
+// wave_rho for n < 2^50 with plain residues and mulmod_f64 (same contract).
+__device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
+  const double nd = (double)n, ninv = 1.0 / nd;
+  const u64 c = lane + 1;
+  auto f = [&](u64 v) { const u64 s = mulmod_f64(v, v, nd, ninv) + c; return s >= n ? s - n : s; };
+  u64 y = 2, x = y, ys = y, acc = 1;
+  bool alive = true;
+  for (u32 r = 1; r <= (1u << 16); r <<= 1) {
+    x = y;
+    for (u32 i = 0; i < r; ++i) y = f(y);
+    for (u32 kk = 0; kk < r; kk += 64) {
+      ys = y;
+      const u32 lim = min(64u, r - kk);
+      for (u32 i = 0; i < lim; ++i) {
+        y = f(y);
+        acc = mulmod_f64(acc, x > y ? x - y : y - x, nd, ninv);
+      }
+      u64 g = alive ? gcd_u64(acc, n) : 1;
+      if (g == n) {                       // the batch overshot: replay it step by step
+        g = 1;
+        for (u32 i = 0; i < lim && g == 1; ++i) {
+          ys = f(ys);
+          const u64 dd = x > ys ? x - ys : ys - x;
+          g = gcd_u64(dd ? dd : n, n);
+        }
+        if (g == n) g = 1;
+        if (g == 1) alive = false;
+      }
+      const u64 found = __ballot(g != 1);
+      if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
+      if (__ballot(alive) == 0) return kRhoFail;
+      if (*stop < k) return kRhoStop;
+    }
+  }
+  return kRhoFail;
 }
 
 // Per-wave factor list in LDS.
@@ -481,7 +603,7 @@ __device__ __forceinline__ void fac_add(WaveFac &F, u64 q, u32 e, u32 lane) {
     else if (F.k < 16) { F.p[F.k] = q; F.e[F.k] = e; ++F.k; }
     else F.bad = 1;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 __device__ __forceinline__ u64 isqrt_dev(u64 m) {
@@ -491,134 +613,175 @@ __device__ __forceinline__ u64 isqrt_dev(u64 m) {
   return s;
 }
 
-// Complete factorization of a (< 2^62) into F; false if the device declines.
-__device__ bool wave_factor(u64 a, const TrialTab &T, WaveFac &F, u32 lane) {
+constexpr int kFacOk = 0, kFacHost = 1, kFacStop = 2;
+
+// Complete factorization of a (< 2^62) into F, given the indices (into T) of the odd
+// primes below 2^16 that divide it. kFacHost if the device declines, kFacStop if a
+// smaller rung settled meanwhile.
+__device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &T, WaveFac &F, u32 lane,
+                           const volatile u32 *stop, u32 k) {
   if (lane == 0) { F.k = 0; F.bad = 0; }
-  __syncthreads();
+  wave_sync();
   u64 n = a;
-  if (!(n & 1)) {
-    const u32 e = (u32)__builtin_ctzll(n);
-    n >>= e;
-    fac_add(F, 2, e, lane);
+  const u32 e2 = (u32)__builtin_ctzll(n);
+  n >>= e2;
+  if (e2) fac_add(F, 2, e2, lane);
+  // Exponent of each sieved prime on its own lane; the cofactor by one exact division.
+  if (F.k + nsmall > 16) return kFacHost;
+  u64 pw = 1;
+  if (lane < nsmall) {
+    const u32 i = small[lane];
+    const u32 q = T.qm[i].x;
+    const u64 qi = T.inv[i], ql = T.lim[i];
+    u64 m = n;
+    u32 e = 0;
+    do { m *= qi; pw *= q; ++e; } while (m * qi <= ql);
+    F.p[F.k + lane] = q;
+    F.e[F.k + lane] = e;
   }
-  bool early = false;
-  for (u32 c0 = 0; c0 < T.n; c0 += 64) {
-    const u64 q0 = T.p[c0];
-    if (q0 * q0 > n) { early = true; break; }
-    const u32 i = c0 + lane;
-    u64 inv = 0, lim = 0;
-    u32 q = 0;
-    bool hit = false;
-    if (i < T.n) {
-      q = T.p[i];
-      inv = T.inv[i];
-      lim = T.lim[i];
-      hit = n * inv <= lim;
-    }
-    u64 mask = __ballot(hit);
-    while (mask) {
-      const int l = __ffsll((long long)mask) - 1;
-      mask &= mask - 1;
-      const u64 qi = __shfl(inv, l, 64), ql = __shfl(lim, l, 64);
-      const u32 qq = __shfl(q, l, 64);
-      u32 e = 0;
-      do { n *= qi; ++e; } while (n * qi <= ql);   // exact division by qq
-      fac_add(F, qq, e, lane);
-    }
-  }
+  wave_sync();
+  if (lane == 0) F.k += nsmall;
+  for (int s = 32; s; s >>= 1) pw *= __shfl_xor(pw, s, 64);
+  n *= inv64(pw);
+  wave_sync();
   if (n > 1) {
-    if (early || n <= 0xFFFFFFFFull) {
-      fac_add(F, n, 1, lane);
-    } else {
-      // Every prime factor of n exceeds 65521, so a part below 2^32 is prime.
-      u64 stack[4];
-      u32 top = 0;
-      stack[top++] = n;
-      while (top) {
-        const u64 m = stack[--top];
-        if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
-        const u64 s = isqrt_dev(m);
-        if (s * s == m) { fac_add(F, s, 2, lane); continue; }
-        const u64 d = wave_rho(m, lane);
-        if (!d || top + 2 > 4) return false;
-        stack[top++] = d;
-        stack[top++] = m / d;
-      }
+    // Every prime factor of n exceeds 65521, so a part below 65537^2 is prime.
+    u64 stack[4];
+    u32 top = 0;
+    stack[top++] = n;
+    while (top) {
+      const u64 m = stack[--top];
+      if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
+      const u64 s = isqrt_dev(m);
+      if (s * s == m) { fac_add(F, s, 2, lane); continue; }
+      const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k) : wave_rho(m, lane, stop, k);
+      if (d == kRhoStop) return kFacStop;
+      if (d == kRhoFail || top + 2 > 4) return kFacHost;
+      stack[top++] = d;
+      stack[top++] = m / d;
     }
   }
-  return F.bad == 0;
+  return F.bad == 0 ? kFacOk : kFacHost;
 }
 
 __device__ __forceinline__ u32 small_mod(u64 x, u32 r, u32 mg, u32 c32) { return mod64(x, r, mg, c32); }
 
-__global__ __launch_bounds__(64) void k_rung(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
-                                             const u8 *__restrict__ flags, TrialTab T, RungOut *__restrict__ out) {
-  __shared__ u128 best[256], nxt[256];
-  __shared__ WaveFac F;
-  const u32 lane = threadIdx.x;
-  const u32 n_ = min(*count, cap);
+// Least witness of rung r for q = pA (A factored in F): u | (pA)^2, u == target (mod r),
+// u <= q, or ~0 if there is none. best/nxt are this wave's residue tables (r entries).
+__device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u32 target, u128 q,
+                              u128 *best, u128 *nxt, u32 lane) {
   const u128 INF = ~(u128)0;
+  for (u32 t = lane; t < r; t += 64) best[t] = t == 1 % r ? (u128)1 : INF;
+  wave_sync();
+  // Absorb p^2, then every prime power of A^2.
+  for (u32 fi = 0; fi <= F.k; ++fi) {
+    const u64 pr = fi == 0 ? p : F.p[fi - 1];
+    const u32 E = fi == 0 ? 2u : 2u * F.e[fi - 1];
+    const u32 pm = small_mod(pr, r, mg, c32);
+    // pm is a unit mod r on every rung that is not skipped: find its inverse.
+    u32 pinv = 0;
+    for (u32 x = lane; x < r; x += 64) if ((pm * x) % r == 1 % r) pinv = x + 1;
+    const u64 bal = __ballot(pinv != 0);
+    pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
+    for (u32 t = lane; t < r; t += 64) {
+      u128 acc = INF, pw = 1;
+      u32 s = t;
+      for (u32 j = 0;; ++j) {
+        const u128 b = best[s];
+        u128 v;
+        if (b != INF && mul_le(b, pw, q, v) && v < acc) acc = v;
+        if (j == E) break;
+        u128 npw;
+        if (!mul_le(pw, (u128)pr, q, npw)) break;
+        pw = npw;
+        s = (s * pinv) % r;
+      }
+      nxt[t] = acc;
+    }
+    wave_sync();
+    for (u32 t = lane; t < r; t += 64) best[t] = nxt[t];
+    wave_sync();
+  }
+  const u128 u = best[target];
+  wave_sync();
+  return u;
+}
+
+__global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
+                                                          const u32 *__restrict__ count, u32 cap,
+                                                          const u8 *__restrict__ flags, TrialTab T,
+                                                          RungOut *__restrict__ out) {
+  __shared__ u128 best[kRungWaves][256], nxt[kRungWaves][256];
+  __shared__ WaveFac F[kRungWaves];
+  __shared__ u16 fac[kRungSlots][kSmallFac];
+  __shared__ u32 nfac[kRungSlots];
+  __shared__ u32 next_k, found_k, host_k, stop_k;
+  __shared__ RungOut wres[kRungWaves];
+  const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const u32 n_ = min(*count, cap);
+  const volatile u32 *stop = &stop_k;
   for (u32 idx = blockIdx.x; idx < n_; idx += gridDim.x) {
     if (!flags[idx]) continue;
     const u64 p = vals[idx];
-    RungOut res{0, 0, 0, 0, kRungNone};
-    u32 r = (u32)((4 - (p & 3)) & 3);
-    if (r == 0) r = 4;
-    for (; r <= 255; r += 4) {
-      const u64 A = (p + r) / 4;
+    u32 r0 = (u32)((4 - (p & 3)) & 3);
+    if (r0 == 0) r0 = 4;
+    const u32 nj = (255 - r0) / 4 + 1;
+    const u64 A0 = (p + r0) / 4;
+    if (tid < kRungSlots) nfac[tid] = 0;
+    if (tid < kRungWaves) wres[tid] = RungOut{0, 0, 0, 0, kRungNone};
+    if (tid == 0) { next_k = 0; found_k = host_k = stop_k = kRungSlots; }
+    __syncthreads();
+    // Sieve the 64 consecutive A_j by every odd prime below 2^16.
+    for (u32 i = tid; i < T.n; i += blockDim.x) {
+      const uint4 c = T.qm[i];
+      const u32 x = mod64(A0, c.x, c.y, c.z);
+      for (u32 j = x ? c.x - x : 0; j < nj; j += c.x) {
+        const u32 s = atomicAdd(&nfac[j], 1u);
+        if (s < kSmallFac) fac[j][s] = (u16)i;
+      }
+    }
+    __syncthreads();
+    for (;;) {
+      u32 k = 0;
+      if (lane == 0) k = atomicAdd(&next_k, 1u);
+      k = __shfl(k, 0, 64);
+      if (k >= nj || k > *stop) break;
+      const u32 r = r0 + 4 * k;
+      const u64 A = A0 + k;
       const u32 mg = r == 1 ? 0xFFFFFFFFu : (u32)(0x100000000ull / r), c32 = (u32)(0x100000000ull % r);
       const u32 qm = (small_mod(p, r, mg, c32) * small_mod(A, r, mg, c32)) % r;
       u32 ga = qm, gb = r;
       while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
       if (r > 1 && ga != 1) continue;
-      const u32 target = qm ? r - qm : 0;
-      if (!wave_factor(A, T, F, lane)) { res.status = kRungHost; break; }
-      const u128 q = (u128)p * A;
-      for (u32 t = lane; t < r; t += 64) best[t] = INF;
-      __syncthreads();
-      if (lane == 0) best[1 % r] = 1;
-      __syncthreads();
-      // Absorb p^2, then every prime power of A^2.
-      for (u32 fi = 0; fi <= F.k; ++fi) {
-        const u64 pr = fi == 0 ? p : F.p[fi - 1];
-        const u32 E = fi == 0 ? 2u : 2u * F.e[fi - 1];
-        const u32 pm = small_mod(pr, r, mg, c32);
-        // pm is a unit mod r on every rung that is not skipped: find its inverse.
-        u32 pinv = 0;
-        for (u32 x = lane; x < r; x += 64) if ((pm * x) % r == 1 % r) pinv = x + 1;
-        const u64 bal = __ballot(pinv != 0);
-        pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
-        for (u32 t = lane; t < r; t += 64) {
-          u128 acc = INF, pw = 1;
-          u32 s = t;
-          for (u32 j = 0;; ++j) {
-            const u128 b = best[s];
-            u128 v;
-            if (b != INF && mul_le(b, pw, q, v) && v < acc) acc = v;
-            if (j == E) break;
-            u128 npw;
-            if (!mul_le(pw, (u128)pr, q, npw)) break;
-            pw = npw;
-            s = (s * pinv) % r;
-          }
-          nxt[t] = acc;
-        }
-        __syncthreads();
-        for (u32 t = lane; t < r; t += 64) best[t] = nxt[t];
-        __syncthreads();
+      const u32 ns = nfac[k];
+      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k);
+      if (fr == kFacStop) break;
+      if (fr == kFacHost) {
+        if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
+        break;
       }
-      const u128 u = best[target];
-      __syncthreads();
-      if (u != INF) {
-        res.A = A;
-        res.u_lo = (u64)u;
-        res.u_hi = (u64)(u >> 64);
-        res.r = r;
-        res.status = kRungFound;
+      const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], nxt[wv], lane);
+      if (u != ~(u128)0) {
+        if (lane == 0) {
+          wres[wv] = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
+          atomicMin(&found_k, k);
+          atomicMin(&stop_k, k);
+        }
         break;
       }
     }
-    if (lane == 0) out[idx] = res;
+    __syncthreads();
+    if (tid == 0) {
+      RungOut res{0, 0, 0, 0, kRungNone};
+      if (host_k < found_k) {
+        res.status = kRungHost;
+      } else if (found_k < kRungSlots) {
+        for (u32 w = 0; w < kRungWaves; ++w)
+          if (wres[w].status == kRungFound && wres[w].r == r0 + 4 * found_k) res = wres[w];
+      }
+      out[idx] = res;
+    }
+    __syncthreads();
   }
 }
 
@@ -686,6 +849,7 @@ struct DeviceCtx::Impl {
   u32 *h_count = nullptr;
   int occupancy_blocks = 0;
   u32 occ_block = 0, occ_bits = 0;
+  u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
 };
 
 DeviceCtx::DeviceCtx() : impl_(new Impl) {}
@@ -700,7 +864,7 @@ DeviceCtx::~DeviceCtx() {
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
     for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_counts,
                     (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one,
-                    (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.p, (void *)m.trial.inv,
+                    (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.qm, (void *)m.trial.inv,
                     (void *)m.trial.lim})
       if (p) (void)hipFree(p);
     if (m.h_hdr) (void)hipHostFree(m.h_hdr);
@@ -728,6 +892,7 @@ int DeviceCtx::open(int device) {
   hipDeviceProp_t p;
   HIP_TRY(hipGetDeviceProperties(&p, device));
   cus_ = p.multiProcessorCount;
+  lds_limit_ = p.sharedMemPerBlock;
   Impl &m = *impl_;
   HIP_TRY(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&m.ev0));
@@ -738,28 +903,29 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
   HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32)));
-  // Trial-division table for stage D: odd primes below 2^16, q^{-1} mod 2^64, floor((2^64-1)/q).
+  // Sieve table for stage D: odd primes q below 2^16 with {q, floor(2^32/q), 2^32 mod q}
+  // (residues of A by q) and q^{-1} mod 2^64, floor((2^64-1)/q) (exact division by q).
   std::vector<uint8_t> comp(65537, 0);
-  std::vector<u32> tp;
+  std::vector<uint4> tq;
   std::vector<u64> tinv, tlim;
   for (u32 i = 3; i <= 65536; i += 2) {
     if (comp[i]) continue;
-    tp.push_back(i);
+    tq.push_back(uint4{i, (u32)(0x100000000ull / i), (u32)(0x100000000ull % i), 0});
     u64 x = i;
     for (int k = 0; k < 6; ++k) x *= 2 - (u64)i * x;
     tinv.push_back(x);
     tlim.push_back(~0ull / i);
     for (u32 j = i * i; j <= 65536; j += 2 * i) comp[j] = 1;
   }
-  u32 *dp = nullptr;
+  uint4 *dq = nullptr;
   u64 *dinv = nullptr, *dlim = nullptr;
-  HIP_TRY(hipMalloc(&dp, tp.size() * 4));
+  HIP_TRY(hipMalloc(&dq, tq.size() * sizeof(uint4)));
   HIP_TRY(hipMalloc(&dinv, tinv.size() * 8));
   HIP_TRY(hipMalloc(&dlim, tlim.size() * 8));
-  HIP_TRY(hipMemcpy(dp, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dq, tq.data(), tq.size() * sizeof(uint4), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dinv, tinv.data(), tinv.size() * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dlim, tlim.data(), tlim.size() * 8, hipMemcpyHostToDevice));
-  m.trial = dev::TrialTab{dp, dinv, dlim, (u32)tp.size()};
+  m.trial = dev::TrialTab{dq, dinv, dlim, (u32)tq.size()};
   return 0;
 }
 
@@ -776,8 +942,15 @@ int DeviceCtx::add_layout(const Layout &L) {
   };
   if (up(&b.lanes, L.lanes.data(), L.lanes.size() * 8) || up(&b.mask, L.mask.data(), L.mask.size() * 4) ||
       up(&b.step, L.step.data(), L.step.size() * 4) || up(&b.magic, L.magic.data(), L.magic.size() * 4) ||
-      up(&b.c32, L.c32.data(), L.c32.size() * 4) || up(&b.off, L.off.data(), L.off.size() * 2))
+      up(&b.c32, L.c32.data(), L.c32.size() * 4))
     return -1;
+  // Offset rows padded to a multiple of 8 entries, so each row starts 16-byte aligned.
+  const u32 ostride = std::max<u32>(8, (L.nf + 7u) & ~7u);
+  const size_t nl = L.lanes.size();
+  std::vector<u16> offp(nl * ostride, 0xFFFFu);
+  for (size_t l = 0; l < nl; ++l)
+    std::memcpy(&offp[l * ostride], &L.off[l * L.nf], L.nf * sizeof(u16));
+  if (up(&b.off, offp.data(), offp.size() * 2)) return -1;
   std::vector<uint4> fc(L.nf);
   for (u32 j = 0; j < L.nf; ++j) fc[j] = make_uint4(L.step[j], L.magic[j], L.c32[j], 0u);
   if (up(&b.fc, fc.data(), fc.size() * sizeof(uint4))) return -1;
@@ -799,8 +972,9 @@ int DeviceCtx::add_layout(const Layout &L) {
   d.cov_small = L.cov_small;
   d.pr_small = L.pr_small;
   d.mask_words = L.mask_words;
-  d.pad = 0;
+  d.ostride = ostride;
   m.layouts.push_back(b);
+  if (ostride > m.off_max) { m.off_max = ostride; m.graph_ok = false; }
   // Mirror the descriptor array on the device.
   if (m.layouts.size() > m.lays_cap) {
     if (m.d_lays) HIP_TRY(hipFree(m.d_lays));
@@ -885,13 +1059,24 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
   const u32 lds_bytes = (tile_max + 31u) & ~31u;
   const u32 lds_words = lds_bytes / 4;
-  const size_t lds = lds_bytes + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
-  if (m.occ_block != cfg.block || m.occ_bits != lds_bytes) {
+  const u32 off_words = m.off_max / 2;
+  const size_t lds = lds_bytes + off_words * 4 + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
+  if (lds > lds_limit_) {
+    err_ = "tile of " + std::to_string(tile_max) + " positions and " + std::to_string(m.off_max) +
+           " filter offsets need " + std::to_string(lds) + " B of LDS, above the device's " +
+           std::to_string(lds_limit_);
+    return -1;
+  }
+  if (m.occ_block != cfg.block || m.occ_bits != lds) {
+    if (lds > 65536) {
+      HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, lds));
     m.occupancy_blocks = std::max(1, nb);
     m.occ_block = cfg.block;
-    m.occ_bits = lds_bytes;
+    m.occ_bits = (u32)lds;
     m.graph_ok = false;
   }
   const u32 pgrid = (u32)((size_t)cus_ * m.occupancy_blocks);
@@ -904,11 +1089,11 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     if (cfg.persistent) {
       dev::k_sieve<true><<<pgrid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
                                                               m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                              m.d_scalars, m.d_scalars + 1, lds_words);
+                                                              m.d_scalars, m.d_scalars + 1, lds_words, off_words);
     } else {
       dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
                                                                  m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                                 m.d_scalars, m.d_scalars + 1, lds_words);
+                                                                 m.d_scalars, m.d_scalars + 1, lds_words, off_words);
     }
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(m.ev2, m.stream));
@@ -919,7 +1104,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
                                                                   m.d_flags, m.d_one);
     HIP_TRY(hipGetLastError());
     if (cfg.device_tail) {
-      dev::k_rung<<<std::max(1, cus_ * 8), 64, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+      dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
                                                                m.d_flags, m.trial, m.d_rung);
       HIP_TRY(hipGetLastError());
     }
@@ -929,7 +1114,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   for (;;) {
     HIP_TRY(hipEventRecord(m.ev0, m.stream));
     if (use_graph) {
-      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds_bytes || m.graph_tail != cfg.device_tail) {
+      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds || m.graph_tail != cfg.device_tail) {
         if (m.graph_exec) { HIP_TRY(hipGraphExecDestroy(m.graph_exec)); m.graph_exec = nullptr; }
         if (m.graph) { HIP_TRY(hipGraphDestroy(m.graph)); m.graph = nullptr; }
         HIP_TRY(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
@@ -942,7 +1127,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
         HIP_TRY(hipGraphInstantiate(&m.graph_exec, m.graph, nullptr, nullptr, 0));
         m.graph_ok = true;
         m.graph_block = cfg.block;
-        m.graph_bits = lds_bytes;
+        m.graph_bits = (u32)lds;
         m.graph_tail = cfg.device_tail;
       }
       HIP_TRY(hipGraphLaunch(m.graph_exec, m.stream));
@@ -1006,7 +1191,7 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_f, ones.data(), n, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
-  dev::k_rung<<<std::max(1, cus_ * 8), 64, 0, m.stream>>>(d_v, d_n, n32, d_f, m.trial, d_o);
+  dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32, d_f, m.trial, d_o);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));

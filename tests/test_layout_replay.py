@@ -71,3 +71,13 @@ def test_replay_matches_oracle_every_layout(replay_bin, name, lo, hi, M, mmax, s
                 assert pos - aft == want["covered"] and aft - sur == want["sieved"], (name, Mp)
             else:
                 assert pos - sur == want["covered"], (name, Mp)
+
+
+def test_stage_d_fp64_mulmod_is_exact(tmp_path):
+    """The device rho's FP64 modular multiply (moduli below 2^50) equals 128-bit integer
+    arithmetic on edge operands and 2 million random ones."""
+    src = os.path.join(HERE, "replay", "mulmod_replay.cpp")
+    exe = str(tmp_path / "mulmod_replay")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", src, "-o", exe], check=True)
+    out = subprocess.run([exe, "2000000", "7"], stdout=subprocess.PIPE, check=True).stdout.decode()
+    assert out.strip() == "ok 2000000", out
