@@ -550,12 +550,23 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
 }
 
 
-// wave_rho for n < 2^50 with plain residues and mulmod_f64 (same contract).
+// x * y mod n on FP64 values that hold integers in [0, n), n < 2^50: mulmod_f64 without
+// the integer conversions, for loops that stay in the FP64 domain.
+__device__ __forceinline__ double mulmod_dd(double a, double b, double nd, double ninv) {
+  const double h = a * b;
+  const double l = __fma_rn(a, b, -h);
+  const double q = floor(h * ninv);
+  double r = __fma_rn(-q, nd, h) + l;
+  r = r < 0 ? r + nd : r;
+  return r >= nd ? r - nd : r;
+}
+
+// wave_rho for n < 2^50 (same contract), the walk kept in FP64: residues below 2^50 and
+// their differences are exact doubles, so only the gcd converts to integers.
 __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
-  const double nd = (double)n, ninv = 1.0 / nd;
-  const u64 c = lane + 1;
-  auto f = [&](u64 v) { const u64 s = mulmod_f64(v, v, nd, ninv) + c; return s >= n ? s - n : s; };
-  u64 y = 2, x = y, ys = y, acc = 1;
+  const double nd = (double)n, ninv = 1.0 / nd, c = (double)(lane + 1);
+  auto f = [&](double v) { const double s = mulmod_dd(v, v, nd, ninv) + c; return s >= nd ? s - nd : s; };
+  double y = 2, x = y, ys = y, acc = 1;
   bool alive = true;
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
     x = y;
@@ -565,14 +576,14 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
       const u32 lim = min(64u, r - kk);
       for (u32 i = 0; i < lim; ++i) {
         y = f(y);
-        acc = mulmod_f64(acc, x > y ? x - y : y - x, nd, ninv);
+        acc = mulmod_dd(acc, fabs(x - y), nd, ninv);
       }
-      u64 g = alive ? gcd_u64(acc, n) : 1;
+      u64 g = alive ? gcd_u64((u64)acc, n) : 1;
       if (g == n) {                       // the batch overshot: replay it step by step
         g = 1;
         for (u32 i = 0; i < lim && g == 1; ++i) {
           ys = f(ys);
-          const u64 dd = x > ys ? x - ys : ys - x;
+          const u64 dd = (u64)fabs(x - ys);
           g = gcd_u64(dd ? dd : n, n);
         }
         if (g == n) g = 1;
@@ -785,6 +796,28 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   }
 }
 
+// Copies one batch's results into pinned host memory with plain stores over the bus
+// (zero-copy), so the host needs one stream synchronisation and no staging copies:
+// the survivor count, the window counters, and per survivor p, window, primality flag
+// and (primes only) the stage-D result.
+__global__ void k_pack(const BatchHdr *__restrict__ hdr, const u64 *__restrict__ counts,
+                       const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
+                       const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
+                       const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, u64 *h_counts,
+                       u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r) {
+  const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  const u32 nc = hdr->nwin * 3, n = min(*surv_count, cap);
+  if (g == 0) h_count[0] = *surv_count;
+  for (u32 i = g; i < nc; i += stride) h_counts[i] = counts[i];
+  for (u32 i = g; i < n; i += stride) {
+    const u8 f = flags[i];
+    h_p[i] = surv_p[i];
+    h_w[i] = surv_w[i];
+    h_f[i] = f;
+    if (with_rung && f) h_r[i] = rung[i];
+  }
+}
+
 // LDS atomic-AND throughput probe: every thread clears bits of its own word for `iters`
 // rounds (consecutive threads on consecutive banks, so no bank conflicts); the word is
 // written out at the end so the work is observable.
@@ -847,6 +880,18 @@ struct DeviceCtx::Impl {
   size_t surv_cap = 0;
   u32 *d_scalars = nullptr;   // [0] survivor count, [1] queue head
   u32 *h_count = nullptr;
+  // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
+  // writes the results into (device-side addresses in the d_h* twins).
+  TileDesc *h_tiles = nullptr;
+  u64 *h_counts = nullptr, *h_surv_p = nullptr;
+  u32 *h_surv_w = nullptr;
+  uint8_t *h_flags = nullptr;
+  dev::RungOut *h_rung = nullptr;
+  u32 *dh_count = nullptr;
+  u64 *dh_counts = nullptr, *dh_surv_p = nullptr;
+  u32 *dh_surv_w = nullptr;
+  uint8_t *dh_flags = nullptr;
+  dev::RungOut *dh_rung = nullptr;
   int occupancy_blocks = 0;
   u32 occ_block = 0, occ_bits = 0;
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
@@ -867,8 +912,9 @@ DeviceCtx::~DeviceCtx() {
                     (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.qm, (void *)m.trial.inv,
                     (void *)m.trial.lim})
       if (p) (void)hipFree(p);
-    if (m.h_hdr) (void)hipHostFree(m.h_hdr);
-    if (m.h_count) (void)hipHostFree(m.h_count);
+    for (void *p : {(void *)m.h_hdr, (void *)m.h_count, (void *)m.h_tiles, (void *)m.h_counts,
+                    (void *)m.h_surv_p, (void *)m.h_surv_w, (void *)m.h_flags, (void *)m.h_rung})
+      if (p) (void)hipHostFree(p);
     if (m.graph_exec) (void)hipGraphExecDestroy(m.graph_exec);
     if (m.graph) (void)hipGraphDestroy(m.graph);
     for (hipEvent_t e : {m.ev0, m.ev1, m.ev2, m.ev3}) if (e) (void)hipEventDestroy(e);
@@ -902,7 +948,8 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipMalloc(&m.d_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
   HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
-  HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32)));
+  HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32), hipHostMallocMapped));
+  HIP_TRY(hipHostGetDevicePointer((void **)&m.dh_count, m.h_count, 0));
   // Sieve table for stage D: odd primes q below 2^16 with {q, floor(2^32/q), 2^32 mod q}
   // (residues of A by q) and q^{-1} mod 2^64, floor((2^64-1)/q) (exact division by q).
   std::vector<uint8_t> comp(65537, 0);
@@ -1016,16 +1063,26 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   if (nt == 0) return 0;
   if (nt > 0xFFFFFFFFull || nwin > 0x55555555u) { err_ = "batch too large"; return -1; }
   // Capacities. Any reallocation invalidates a captured graph.
+  // Pinned host buffer with its device-side address (for k_pack's stores).
+  auto pinned = [&](void **h, void **d, size_t bytes) -> int {
+    if (*h) HIP_TRY(hipHostFree(*h));
+    *h = nullptr;
+    HIP_TRY(hipHostMalloc(h, bytes, hipHostMallocMapped));
+    if (d) HIP_TRY(hipHostGetDevicePointer(d, *h, 0));
+    return 0;
+  };
   if (nt > m.tiles_cap) {
     if (m.d_tiles) HIP_TRY(hipFree(m.d_tiles));
     m.tiles_cap = std::max<size_t>(nt, 2 * m.tiles_cap);
     HIP_TRY(hipMalloc(&m.d_tiles, m.tiles_cap * sizeof(TileDesc)));
+    if (pinned((void **)&m.h_tiles, nullptr, m.tiles_cap * sizeof(TileDesc))) return -1;
     m.graph_ok = false;
   }
   if (nwin > m.win_cap) {
     if (m.d_counts) HIP_TRY(hipFree(m.d_counts));
     m.win_cap = std::max<size_t>(nwin, 2 * m.win_cap);
     HIP_TRY(hipMalloc(&m.d_counts, m.win_cap * 3 * sizeof(u64)));
+    if (pinned((void **)&m.h_counts, (void **)&m.dh_counts, m.win_cap * 3 * sizeof(u64))) return -1;
     m.graph_ok = false;
   }
   auto grow_surv = [&](size_t need) -> int {
@@ -1040,6 +1097,11 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     HIP_TRY(hipMalloc(&m.d_rung, cap * sizeof(dev::RungOut)));
     HIP_TRY(hipMalloc(&m.d_surv_w, cap * sizeof(u32)));
     HIP_TRY(hipMalloc(&m.d_flags, cap));
+    if (pinned((void **)&m.h_surv_p, (void **)&m.dh_surv_p, cap * sizeof(u64)) ||
+        pinned((void **)&m.h_surv_w, (void **)&m.dh_surv_w, cap * sizeof(u32)) ||
+        pinned((void **)&m.h_flags, (void **)&m.dh_flags, cap) ||
+        pinned((void **)&m.h_rung, (void **)&m.dh_rung, cap * sizeof(dev::RungOut)))
+      return -1;
     m.surv_cap = cap;
     m.graph_ok = false;
     return 0;
@@ -1051,7 +1113,8 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   m.h_hdr->has_sieve = has_sieve ? 1 : 0;
   m.h_hdr->pad = 0;
   HIP_TRY(hipMemcpyAsync(m.d_hdr, m.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
-  HIP_TRY(hipMemcpyAsync(m.d_tiles, tiles.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
+  std::memcpy(m.h_tiles, tiles.data(), nt * sizeof(TileDesc));
+  HIP_TRY(hipMemcpyAsync(m.d_tiles, m.h_tiles, nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
 
   // One LDS byte per position, sized to the batch's largest tile (rounded to whole
   // 32-position chunks): short tiles then leave room for more resident workgroups.
@@ -1108,6 +1171,11 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
                                                                m.d_flags, m.trial, m.d_rung);
       HIP_TRY(hipGetLastError());
     }
+    dev::k_pack<<<std::max(1, cus_), 256, 0, m.stream>>>(m.d_hdr, m.d_counts, m.d_scalars, (u32)m.surv_cap,
+                                                         m.d_surv_p, m.d_surv_w, m.d_flags, m.d_rung,
+                                                         cfg.device_tail ? 1u : 0u, m.dh_count, m.dh_counts,
+                                                         m.dh_surv_p, m.dh_surv_w, m.dh_flags, m.dh_rung);
+    HIP_TRY(hipGetLastError());
     return 0;
   };
   const bool use_graph = cfg.graph && cfg.persistent;
@@ -1135,7 +1203,6 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
       if (enqueue(true)) return -1;
     }
     HIP_TRY(hipEventRecord(m.ev3, m.stream));
-    HIP_TRY(hipMemcpyAsync(m.h_count, m.d_scalars, sizeof(u32), hipMemcpyDeviceToHost, m.stream));
     HIP_TRY(hipStreamSynchronize(m.stream));
     float all = 0, sv = 0;
     HIP_TRY(hipEventElapsedTime(&all, m.ev0, m.ev3));
@@ -1149,26 +1216,20 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     ++out.replays;
     if (grow_surv(cnt)) return -1;
   }
+  // Everything is in the pinned mirrors now (k_pack ran before the synchronisation).
   const u32 cnt = *m.h_count;
-  std::vector<u64> raw(nwin * 3);
-  HIP_TRY(hipMemcpyAsync(raw.data(), m.d_counts, nwin * 3 * sizeof(u64), hipMemcpyDeviceToHost, m.stream));
-  out.surv_p.resize(cnt);
-  out.surv_w.resize(cnt);
+  const u64 *raw = m.h_counts;
+  out.surv_p.assign(m.h_surv_p, m.h_surv_p + cnt);
+  out.surv_w.assign(m.h_surv_w, m.h_surv_w + cnt);
   out.surv_prime.resize(cnt);
-  if (cnt) {
-    HIP_TRY(hipMemcpyAsync(out.surv_p.data(), m.d_surv_p, cnt * sizeof(u64), hipMemcpyDeviceToHost, m.stream));
-    HIP_TRY(hipMemcpyAsync(out.surv_w.data(), m.d_surv_w, cnt * sizeof(u32), hipMemcpyDeviceToHost, m.stream));
-    HIP_TRY(hipMemcpyAsync(out.surv_prime.data(), m.d_flags, cnt, hipMemcpyDeviceToHost, m.stream));
-    if (cfg.device_tail) {
-      static_assert(sizeof(dev::RungOut) == sizeof(RungResult), "RungOut mirrors RungResult");
-      out.rung.resize(cnt);
-      HIP_TRY(hipMemcpyAsync(out.rung.data(), m.d_rung, cnt * sizeof(RungResult), hipMemcpyDeviceToHost,
-                             m.stream));
-    }
+  for (u32 i = 0; i < cnt; ++i) out.surv_prime[i] = m.h_flags[i] != 0;   // pending after every base == prime
+  if (cfg.device_tail) {
+    static_assert(sizeof(dev::RungOut) == sizeof(RungResult), "RungOut mirrors RungResult");
+    out.rung.resize(cnt);
+    for (u32 i = 0; i < cnt; ++i)
+      if (out.surv_prime[i]) std::memcpy(&out.rung[i], &m.h_rung[i], sizeof(RungResult));
   }
-  HIP_TRY(hipStreamSynchronize(m.stream));
   for (u32 w = 0; w < nwin; ++w) out.counts[w] = WinCounts{raw[w * 3], raw[w * 3 + 1], raw[w * 3 + 2]};
-  for (uint8_t &f : out.surv_prime) f = f != 0;   // pending after every base == prime
   return 0;
 }
 
