@@ -550,48 +550,77 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
 }
 
 
-// x * y mod n on FP64 values that hold integers in [0, n), n < 2^50: mulmod_f64 without
-// the integer conversions, for loops that stay in the FP64 domain.
-__device__ __forceinline__ double mulmod_dd(double a, double b, double nd, double ninv) {
+// a * b mod n on FP64 values holding integers, n < 2^50, with signed residues:
+// q = rint(a*b/n) makes r = a*b - q*n exact (FMA residual, one exact FMA) with
+// |r| <= (1/2 + 3*2^-53*|a*b|/n) * n. So |a|, |b| < n + 128 give |r| < 7n/8 (the walk
+// needs no correction), and |a| < n, |b| < 2n + 256 give |r| < 5n/4 (one correction).
+__device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, double ninv) {
   const double h = a * b;
   const double l = __fma_rn(a, b, -h);
-  const double q = floor(h * ninv);
-  double r = __fma_rn(-q, nd, h) + l;
-  r = r < 0 ? r + nd : r;
-  return r >= nd ? r - nd : r;
+  const double q = rint(h * ninv);
+  return __fma_rn(-q, nd, h) + l;
 }
 
-// wave_rho for n < 2^50 (same contract), the walk kept in FP64: residues below 2^50 and
-// their differences are exact doubles, so only the gcd converts to integers.
+constexpr int kRhoWalks = 2;   // independent walks per lane (interleaved for ILP)
+
+// wave_rho for n < 2^50 (same contract): 64 x kRhoWalks walks x -> x^2 + c with
+// c = lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts to
+// integers, and gcd(a, n) does not depend on which representative of a mod n it sees.
 __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
-  const double nd = (double)n, ninv = 1.0 / nd, c = (double)(lane + 1);
-  auto f = [&](double v) { const double s = mulmod_dd(v, v, nd, ninv) + c; return s >= nd ? s - nd : s; };
-  double y = 2, x = y, ys = y, acc = 1;
-  bool alive = true;
+  const double nd = (double)n, ninv = 1.0 / nd;
+  double c[kRhoWalks], y[kRhoWalks], x[kRhoWalks], ys[kRhoWalks], acc[kRhoWalks];
+  bool alive[kRhoWalks];
+#pragma unroll
+  for (int w = 0; w < kRhoWalks; ++w) {
+    c[w] = (double)(lane + 1 + 64 * w);
+    y[w] = x[w] = ys[w] = 2;
+    acc[w] = 1;
+    alive[w] = true;
+  }
+  auto f = [&](double v, double cc) { return mulmod_sgn(v, v, nd, ninv) + cc; };
+  auto red = [&](double r) {
+    r = r >= nd ? r - nd : r;
+    return r <= -nd ? r + nd : r;
+  };
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
-    x = y;
-    for (u32 i = 0; i < r; ++i) y = f(y);
+#pragma unroll
+    for (int w = 0; w < kRhoWalks; ++w) x[w] = y[w];
+    for (u32 i = 0; i < r; ++i) {
+#pragma unroll
+      for (int w = 0; w < kRhoWalks; ++w) y[w] = f(y[w], c[w]);
+    }
     for (u32 kk = 0; kk < r; kk += 64) {
-      ys = y;
       const u32 lim = min(64u, r - kk);
+#pragma unroll
+      for (int w = 0; w < kRhoWalks; ++w) ys[w] = y[w];
       for (u32 i = 0; i < lim; ++i) {
-        y = f(y);
-        acc = mulmod_dd(acc, fabs(x - y), nd, ninv);
-      }
-      u64 g = alive ? gcd_u64((u64)acc, n) : 1;
-      if (g == n) {                       // the batch overshot: replay it step by step
-        g = 1;
-        for (u32 i = 0; i < lim && g == 1; ++i) {
-          ys = f(ys);
-          const u64 dd = (u64)fabs(x - ys);
-          g = gcd_u64(dd ? dd : n, n);
+#pragma unroll
+        for (int w = 0; w < kRhoWalks; ++w) {
+          y[w] = f(y[w], c[w]);
+          acc[w] = red(mulmod_sgn(acc[w], x[w] - y[w], nd, ninv));
         }
-        if (g == n) g = 1;
-        if (g == 1) alive = false;
+      }
+      u64 g = 1;
+      bool any = false;
+#pragma unroll
+      for (int w = 0; w < kRhoWalks; ++w) {
+        u64 gw = alive[w] ? gcd_u64((u64)fabs(acc[w]), n) : 1;
+        if (gw == n) {                    // the batch overshot: replay it step by step
+          gw = 1;
+          for (u32 i = 0; i < lim && gw == 1; ++i) {
+            ys[w] = f(ys[w], c[w]);
+            const u64 dd = (u64)fabs(x[w] - ys[w]);
+            gw = gcd_u64(dd ? dd : n, n);
+          }
+          if (gw == n) gw = 1;
+          if (gw == 1) alive[w] = false;
+        }
+        if (g == 1) g = gw;
+        any |= alive[w];
       }
       const u64 found = __ballot(g != 1);
       if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
-      if (__ballot(alive) == 0) return kRhoFail;
+      if (__ballot(any) == 0) return kRhoFail;
       if (*stop < k) return kRhoStop;
     }
   }

@@ -1,8 +1,13 @@
-// mulmod_replay.cpp -- test-only CPU replay of the stage-D FP64 modular multiply
-// (x * y mod n for x, y < n < 2^50: product and FMA residual, floor quotient estimate,
-// one exact FMA for h - q*n, one correction each way) against 128-bit integer
-// arithmetic. IEEE-754 binary64 with a fused multiply-add is what the device executes,
-// so std::fma gives the same results bit for bit.
+// mulmod_replay.cpp -- test-only CPU replay of the stage-D FP64 modular multiplies
+// against 128-bit integer arithmetic:
+//   mulmod_f64   x * y mod n for x, y < n < 2^50: product and FMA residual, floor
+//                quotient estimate, one exact FMA for h - q*n, one correction each way;
+//   mulmod_sgn   the signed-residue form rho walks in: a * b with q = rint(a*b/n) gives
+//                r == a*b (mod n) with |r| <= (1/2 + 3*2^-53*|a*b|/n) * n, exactly, for
+//                |a*b| < 2^101 -- so |r| < n for |a|, |b| < n + 64 (the walk) and
+//                |r| < 1.25 n for |a| < n, |b| < 2n + 128 (the product of differences).
+// IEEE-754 binary64 with a fused multiply-add is what the device executes, so std::fma
+// and std::rint (round to nearest even) give the same results bit for bit.
 //
 // usage: mulmod_replay TRIALS SEED     prints "ok TRIALS" or the first mismatch.
 #include <cmath>
@@ -23,6 +28,31 @@ static u64 mulmod_f64(u64 x, u64 y, double nd, double ninv) {
   if (r < 0) r += nd;
   if (r >= nd) r -= nd;
   return (u64)r;
+}
+
+static double mulmod_sgn(double a, double b, double nd, double ninv) {
+  const double h = a * b;
+  const double l = std::fma(a, b, -h);
+  const double q = std::rint(h * ninv);
+  return std::fma(-q, nd, h) + l;
+}
+
+typedef __int128 i128;
+
+// r must be an integer-valued double with r == a*b (mod n) and |r| < bound * n.
+static int check_sgn(long long a, long long b, u64 n, double bound) {
+  const double nd = (double)n, ninv = 1.0 / nd;
+  const double r = mulmod_sgn((double)a, (double)b, nd, ninv);
+  const i128 ri = (i128)r;
+  const bool integral = (double)ri == r;
+  const i128 diff = (i128)a * b - ri;
+  const bool congruent = diff % (i128)n == 0;
+  const bool bounded = std::fabs(r) < bound * nd;
+  if (!integral || !congruent || !bounded) {
+    printf("signed mismatch a=%lld b=%lld n=%llu r=%.1f\n", a, b, (unsigned long long)n, r);
+    return 1;
+  }
+  return 0;
 }
 
 static u64 state;
@@ -60,6 +90,31 @@ int main(int argc, char **argv) {
     if (n >= top) n = top - 1;
     const u64 x = next() % n, y = (t & 7) == 0 ? n - 1 - (next() & 255) % n : next() % n;
     if (check(x, y, n)) return 1;
+  }
+  // Signed residues: the walk (|a|, |b| < n + 64) and the difference products.
+  for (u64 n : ns) {
+    const long long m = (long long)n;
+    for (long long a : {0LL, 1LL, -1LL, m - 1, 1 - m, m + 63, -m - 63, m / 2, -m / 2})
+      for (long long b : {0LL, 1LL, -1LL, m - 1, 1 - m, m + 63, -m - 63, m / 2 + 1, -m / 2 - 1})
+        if (check_sgn(a, b, n, 1.0)) return 1;
+    for (long long a : {1LL, m - 1, 1 - m, m / 3})
+      for (long long b : {2 * m + 127, -2 * m - 127, 2 * m - 1, m + 1})
+        if (check_sgn(a, b, n, 1.25)) return 1;
+  }
+  for (u64 t = 0; t < trials; ++t) {
+    const int bits = 33 + (int)(next() % 18);
+    u64 n = (next() >> (64 - bits)) | (1ull << (bits - 1)) | 1;
+    if (n >= top) n = top - 1;
+    const long long m = (long long)n;
+    const long long a = (long long)(next() % (2 * n + 127)) - m - 63;
+    if ((t & 1) == 0) {
+      const long long b = (long long)(next() % (2 * n + 127)) - m - 63;
+      if (check_sgn(a, b, n, 1.0)) return 1;
+    } else {
+      const long long x = (long long)(next() % (2 * n - 1)) - m + 1;
+      const long long d = (long long)(next() % (4 * n + 255)) - 2 * m - 127;
+      if (check_sgn(x, d, n, 1.25)) return 1;
+    }
   }
   printf("ok %llu\n", (unsigned long long)trials);
   return 0;
