@@ -9,8 +9,9 @@ lane order + SUMMARY), which must be byte-identical to the reference scanner's r
 timed region; the record is produced inside it.
 
 Two forms are timed:
-  batched  the interval cut by the autotuner into tiles and run as one persistent
-           work-queue launch (the headline `value`)
+  batched  the interval cut into tiles (layout and launch shape from the autotuner) and
+           run as one persistent work-queue launch replayed from a hipGraph (the
+           headline `value`)
   plain    4096 windows of 2^22 integers, one launch (and one host round trip) per
            window, records merged by the lane key
 
@@ -19,7 +20,11 @@ Also reported: the marking kernel against the LDS roofline (exact marks from the
 layout tables, kernel time from HIP events on its own stream), and the reference
 scanner built from source (oracle/_ref/rung_scan3) timed on this host's cores.
 
+Two more configurations ride along (skippable): the batched 512-window set (--no-c4) and
+a real-time replay of the serving trace (--c5-seconds 0 to skip).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--no-cpu]
+                       [--no-c4] [--c5-seconds S]
 """
 import argparse
 import importlib.util
@@ -138,6 +143,36 @@ def c4_bench(es, reps=5):
             "by_wheel": res}
 
 
+def c5_bench(es, seconds):
+    """Serving configuration: the Poisson request trace (200 req/s, seed 1, start
+    log-uniform over [1e6, 1.8e19], length log-uniform over [1e4, 1e10]) replayed in real
+    time for its first `seconds` through the continuous-batching server (wheel 2042040,
+    the GPU filter table, admission budget 80% of device memory). Latency is arrival to
+    record complete; the first requests' records must equal the golden records."""
+    gdir = os.path.join(REPO, "tests", "golden")
+    rows = [l.split() for l in open(os.path.join(gdir, "c5_trace.txt")) if not l.startswith("#")]
+    reqs = [(float(a), int(lo), int(hi)) for a, lo, hi in rows if float(a) < seconds]
+    gold = split_records(open(os.path.join(gdir, "c5", "records_first400.txt")).read())
+    sc = es.Scanner(wheel=2042040, filters=es.F204, device=0)
+    sample = [(lo, min(hi, lo + (1 << 32))) for _, lo, hi in reqs[:64]]
+    sc.autotune(sample)
+    recs, infos, st = es.serve_trace(sc, reqs, realtime=True, hbm_fraction=0.8)
+    sc.close()
+    lat = [i["done_ms"] - i["arrival_ms"] for i in infos]
+    k = min(len(gold), len(recs))
+    total = sum(hi - lo for _, lo, hi in reqs)
+    return {"workload": "Poisson 200 req/s (seed 1), first %.0f s of the trace: %d requests, %d integers"
+                        % (seconds, len(reqs), total),
+            "sustained": total / (st["wall_ms"] * 1e-3), "unit": "integers/s",
+            "offered": total / seconds,
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+            "records_identical_first": k, "records_identical": recs[:k] == gold[:k],
+            "batches": st["batches"], "busy_ms": st["busy_ms"], "wall_ms": st["wall_ms"],
+            "max_active": st["max_active"], "max_batch_requests": st["max_batch_requests"],
+            "hbm_budget_gb": st["budget"] / 1e9, "resident_gb": st["resident"] / 1e9,
+            "peak_reserved_gb": st["peak_reserved"] / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +181,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched 512-window configuration")
+    ap.add_argument("--c5-seconds", type=float, default=10.0,
+                    help="seconds of the serving trace to replay in real time (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -255,6 +292,8 @@ def main():
     }
     if not args.no_c4:
         out["c4"] = c4_bench(es)
+    if args.c5_seconds > 0:
+        out["c5"] = c5_bench(es, args.c5_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(os.cpu_count() or 1, 16))
     print(json.dumps(out))

@@ -6,7 +6,11 @@
 
 #include "../../include/es_sieve.h"
 #include "es_scan.h"
+#include "es_serve.h"
 #include "es_tail.h"
+
+#include <chrono>
+#include <thread>
 
 struct es_scanner {
   es::Scanner sc;
@@ -273,6 +277,159 @@ int es_wheel_classes(const es_scanner *s, uint64_t *out, size_t cap, size_t *n) 
   const std::vector<es::u64> &c = s->sc.plan().wheel.classes;
   *n = c.size();
   for (size_t i = 0; i < c.size() && i < cap && out; ++i) out[i] = c[i];
+  return ES_OK;
+}
+
+void es_serve_config_default(es_serve_config *c) {
+  if (!c) return;
+  const es::ServeConfig d;
+  c->slice = d.slice;
+  c->batch_ints = d.batch_ints;
+  c->batch_windows = d.batch_windows;
+  c->slots = d.slots;
+  c->hbm_fraction = d.hbm_fraction;
+  c->hbm_budget = d.hbm_budget;
+  c->emit_survivors = d.emit_survivors ? 1 : 0;
+}
+
+struct es_server {
+  es::Server srv;
+  es_server(es::Scanner &sc, const es::ServeConfig &cfg) : srv(sc, cfg) {}
+};
+
+static es::ServeConfig serve_config(const es_serve_config *c) {
+  es::ServeConfig k;
+  if (c) {
+    k.slice = c->slice;
+    k.batch_ints = c->batch_ints;
+    k.batch_windows = c->batch_windows;
+    k.slots = c->slots;
+    k.hbm_fraction = c->hbm_fraction;
+    k.hbm_budget = c->hbm_budget;
+    k.emit_survivors = c->emit_survivors != 0;
+  }
+  return k;
+}
+
+static void fill_info(const es::RequestInfo &i, es_request_info *o) {
+  if (!o) return;
+  o->arrival_ms = i.arrival_ms;
+  o->admit_ms = i.admit_ms;
+  o->done_ms = i.done_ms;
+  o->batches = i.batches;
+  o->reserved = i.reserved;
+  o->exit_code = i.exit_code;
+}
+
+static void fill_stats(const es::ServeStats &s, es_serve_stats *o) {
+  if (!o) return;
+  o->submitted = s.submitted;
+  o->completed = s.completed;
+  o->refused = s.refused;
+  o->batches = s.batches;
+  o->slices = s.slices;
+  o->ints = s.ints;
+  o->budget = s.budget;
+  o->resident = s.resident;
+  o->peak_reserved = s.peak_reserved;
+  o->max_active = s.max_active;
+  o->max_batch_requests = s.max_batch_requests;
+  o->max_queued = s.max_queued;
+  o->busy_ms = s.busy_ms;
+  o->wall_ms = 0;
+}
+
+int es_server_open(es_scanner *s, const es_serve_config *cfg, es_server **out) {
+  if (!s || !out) return fail(ES_ERR_ARG, "null argument");
+  *out = nullptr;
+  es_server *v = new es_server(s->sc, serve_config(cfg));
+  std::string msg;
+  if (v->srv.start(msg)) {
+    delete v;
+    return fail(ES_ERR_DEVICE, msg);
+  }
+  *out = v;
+  return ES_OK;
+}
+
+int es_server_submit(es_server *v, uint64_t lo, uint64_t hi, uint64_t *id) {
+  if (!v || !id) return fail(ES_ERR_ARG, "null argument");
+  *id = v->srv.submit(lo, hi);
+  return ES_OK;
+}
+
+int es_server_wait(es_server *v, uint64_t id, double timeout_ms, char **record, size_t *len,
+                   es_request_info *info) {
+  if (!v || !record) return fail(ES_ERR_ARG, "null argument");
+  *record = nullptr;
+  std::string rec;
+  es::RequestInfo ri;
+  const int rc = v->srv.wait(id, timeout_ms, rec, ri);
+  if (rc == 1) return 1;
+  if (rc < 0) {
+    const es::ServeStats st = v->srv.stats();
+    return fail(st.error ? ES_ERR_DEVICE : ES_ERR_ARG, st.error ? st.error_msg : "unknown request id");
+  }
+  *record = (char *)malloc(rec.size() + 1);
+  if (!*record) return fail(ES_ERR_ARG, "out of host memory");
+  memcpy(*record, rec.data(), rec.size());
+  (*record)[rec.size()] = 0;
+  if (len) *len = rec.size();
+  fill_info(ri, info);
+  return ES_OK;
+}
+
+int es_server_stats(es_server *v, es_serve_stats *st) {
+  if (!v || !st) return fail(ES_ERR_ARG, "null argument");
+  fill_stats(v->srv.stats(), st);
+  return ES_OK;
+}
+
+void es_server_close(es_server *v) { delete v; }
+
+int es_serve_trace(es_scanner *s, const es_serve_config *cfg, const double *arrival_s, const uint64_t *lo,
+                   const uint64_t *hi, size_t n, int realtime, char **text, size_t *rec_len,
+                   es_request_info *info, es_serve_stats *stats) {
+  if (!s || !text || (n && (!lo || !hi || (realtime && !arrival_s)))) return fail(ES_ERR_ARG, "null argument");
+  *text = nullptr;
+  es::Server srv(s->sc, serve_config(cfg));
+  std::string msg;
+  if (srv.start(msg)) return fail(ES_ERR_DEVICE, msg);
+  std::vector<uint64_t> ids(n);
+  const double t_start = srv.now_ms();
+  for (size_t i = 0; i < n; ++i) {
+    if (realtime) {
+      const double due = t_start + arrival_s[i] * 1e3;
+      const double wait = due - srv.now_ms();
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+    }
+    ids[i] = srv.submit(lo[i], hi[i]);
+  }
+  std::string all;
+  double first = 1e300, last = 0;
+  for (size_t i = 0; i < n; ++i) {
+    std::string rec;
+    es::RequestInfo ri;
+    if (srv.wait(ids[i], -1, rec, ri) != 0) {
+      const es::ServeStats st = srv.stats();
+      srv.stop();
+      return fail(ES_ERR_DEVICE, st.error ? st.error_msg : "request lost");
+    }
+    if (rec_len) rec_len[i] = rec.size();
+    all += rec;
+    fill_info(ri, info ? info + i : nullptr);
+    first = std::min(first, ri.arrival_ms);
+    last = std::max(last, ri.done_ms);
+  }
+  srv.stop();
+  if (stats) {
+    fill_stats(srv.stats(), stats);
+    stats->wall_ms = n ? last - first : 0;
+  }
+  *text = (char *)malloc(all.size() + 1);
+  if (!*text) return fail(ES_ERR_ARG, "out of host memory");
+  memcpy(*text, all.data(), all.size());
+  (*text)[all.size()] = 0;
   return ES_OK;
 }
 

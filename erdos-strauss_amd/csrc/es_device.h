@@ -82,6 +82,9 @@ class DeviceCtx {
   const std::string &last_error() const { return err_; }
   int device() const { return device_; }
   size_t layout_bytes() const { return layout_bytes_; }
+  // Device memory: free and total bytes (hipMemGetInfo) and what this context holds.
+  int mem_info(size_t &free_b, size_t &total_b);
+  size_t resident_bytes() const;
   std::string device_name() const;
   int cu_count() const { return cus_; }
 

@@ -1289,6 +1289,20 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   return 0;
 }
 
+int DeviceCtx::mem_info(size_t &free_b, size_t &total_b) {
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  return 0;
+}
+
+size_t DeviceCtx::resident_bytes() const {
+  const Impl &m = *impl_;
+  const size_t per_surv = sizeof(u64) * 2 + sizeof(u32) + 1 + sizeof(dev::RungOut);
+  return layout_bytes_ + m.trial.n * (sizeof(uint4) + 16) + m.tiles_cap * sizeof(TileDesc) +
+         m.win_cap * 3 * sizeof(u64) + m.surv_cap * per_surv;
+}
+
 int DeviceCtx::lds_peak(double &ops_per_s) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }

@@ -413,7 +413,6 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
   if (device_pass(wins, out, br, tm, msg)) return -1;
   const double t2 = now_ms();
   const double t0 = t2 - tm.total_ms;
-  const u64 M = plan_.M;
   std::vector<size_t> prime_idx;
   for (size_t i = 0; i < br.surv_p.size(); ++i)
     if (br.surv_prime[i]) prime_idx.push_back(i);
@@ -443,22 +442,14 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
   tm.tail_ms = t3 - t2;
 
   // Emission order.
-  const std::vector<u64> &cls = plan_.wheel.classes;
   struct Keyed { u64 k1, k2, p; size_t j; };
   std::vector<std::vector<Keyed>> per(wins.size());
   for (size_t j = 0; j < prime_idx.size(); ++j) {
     const size_t i = prime_idx[j];
     const u64 p = br.surv_p[i];
     const u32 w = br.surv_w[i];
-    const u64 rho = p % M;
-    const u64 rank = (u64)(std::lower_bound(cls.begin(), cls.end(), rho) - cls.begin());
-    u64 k1 = rank, k2 = 0;
-    if (order == Order::Round) {
-      const u64 lo = wins[w].lo;
-      const u64 first = lo + (rho + M - lo % M) % M;
-      k1 = ((p - first) / M) / round_seg;
-      k2 = rank;
-    }
+    u64 k1, k2;
+    emit_key(p, wins[w].lo, k1, k2);
     per[w].push_back(Keyed{k1, k2, p, j});
   }
   for (size_t w = 0; w < wins.size(); ++w) {
@@ -479,6 +470,31 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
     }
   }
   tm.total_ms = now_ms() - t0;
+  return 0;
+}
+
+void Scanner::emit_key(u64 p, u64 lo, u64 &k1, u64 &k2) const {
+  const u64 M = plan_.M;
+  const std::vector<u64> &cls = plan_.wheel.classes;
+  const u64 rho = p % M;
+  const u64 rank = (u64)(std::lower_bound(cls.begin(), cls.end(), rho) - cls.begin());
+  k1 = rank;
+  k2 = 0;
+  if (order == Order::Round) {
+    const u64 first = lo + (rho + M - lo % M) % M;
+    k1 = ((p - first) / M) / round_seg;
+    k2 = rank;
+  }
+}
+
+int Scanner::warm_layouts(std::string &msg) {
+  std::vector<u64> want;
+  if (tune_.fixed_Mp) want.push_back(tune_.fixed_Mp);
+  for (const auto &e : tune_.table) want.push_back(e.second);
+  if (want.empty())
+    for (int lw = 10; lw <= 40; ++lw) want.push_back(choose_layout(1ull << lw));
+  for (u64 Mp : want)
+    if (ensure_layout(Mp, msg) < 0) return -1;
   return 0;
 }
 

@@ -83,6 +83,12 @@ class Scanner {
   const Layout *host_layout(u64 Mp, std::string &msg);
   const Layout *layout(u64 Mp) const;
 
+  // Emission key (k1, k2, then p) of the certificate line of prime p in a record that
+  // starts at lo, for the current order.
+  void emit_key(u64 p, u64 lo, u64 &k1, u64 &k2) const;
+  // Uploads every layout the tuner can pick, so no later batch pays for an upload.
+  int warm_layouts(std::string &msg);
+
   // The full record text: lines (SURVIVOR lines dropped if !emit_survivors) + SUMMARY.
   std::string record_text(const WindowRecord &r, bool emit_survivors) const;
 

@@ -1,13 +1,14 @@
 // es_sieve_main.cpp -- `es_sieve`, the drop-in for rung_scan3 (and sieve_cuda) under a
 // shard driver: the same argv, the same stdout (certificate lines then one SUMMARY),
-// the same exit codes (0 clean, 2 survivors, 1 refusal/config error). The funnel runs
-// on the GPU; stage D/E on the host.
+// the same exit codes (0 clean, 2 survivors, 1 refusal/config error). The funnel and
+// stage D run on the GPU; stage E, and any prime the device declines, on the host.
 //
 //   es_sieve LO HI [--mmax N] [--wheel M] [--seg N] [--threads N] [--shard i/n]
 //                  [--progress N] [--filters FILE] [--sieve P] [--no-emit-survivors]
 //            GPU options (additive, output-invariant):
 //                  [--device D] [--block B] [--tile-bits N] [--layout Mp|auto]
 //                  [--plain] [--chunk N] [--order lane|round] [--profile] [--quiet-time]
+//                  [--host-tail] [--graph]
 //   es_sieve --verify | --dump-classes [--wheel M] | --check-direct LO N
 //            | --check-factor [N] | --check-primality LO N [--device D]
 //
@@ -260,7 +261,7 @@ struct Cli {
   u64 lo = 0, hi = 0, M = 2042040, mmax = 10000, seg = 262144, shard = 0, nshard = 1, progress = 0, sieve = 0;
   int threads = 0, device = 0;
   const char *filters = nullptr;
-  bool emit_surv = true, plain = false, profile = false, quiet_time = false;
+  bool emit_surv = true, plain = false, profile = false, quiet_time = false, host_tail = false, graph = false;
   u32 block = 256, tile_bits = 32768;
   u64 layout = 0, chunk = 1ull << 40;
   Order order = Order::Lane;
@@ -290,7 +291,7 @@ int main(int argc, char **argv) {
             "                 [--shard i/n] [--progress N] [--filters FILE] [--sieve P]\n"
             "                 [--no-emit-survivors] [--device D] [--block B] [--tile-bits N]\n"
             "                 [--layout Mp|auto] [--plain] [--chunk N] [--order lane|round]\n"
-            "                 [--profile] [--quiet-time]\n"
+            "                 [--profile] [--quiet-time] [--host-tail] [--graph]\n"
             "       %s --verify | --dump-classes [--wheel M] | --check-direct LO N\n"
             "          | --check-factor [N] | --check-primality LO N [--device D]\n"
             "Verification sieve on an MI355X: certifies 4/p solvable for every prime in [LO, HI).\n"
@@ -329,6 +330,8 @@ int main(int argc, char **argv) {
       else if (!strcmp(argv[i], "round")) o.order = Order::Round;
       else { fprintf(stderr, "FATAL: --order takes lane or round\n"); return 1; }
     } else if (!strcmp(a, "--profile")) o.profile = true;
+    else if (!strcmp(a, "--host-tail")) o.host_tail = true;
+    else if (!strcmp(a, "--graph")) o.graph = true;
     else if (!strcmp(a, "--quiet-time")) o.quiet_time = true;
     else {
       fprintf(stderr, "FATAL: unknown option '%s'\n", a);
@@ -366,7 +369,9 @@ int main(int argc, char **argv) {
   sc.tuning().launch.persistent = !o.plain;
   sc.tuning().launch.block = o.block;
   sc.tuning().launch.max_bits = o.tile_bits;
-  if (o.hi > o.lo && sc.open_device(o.device, msg)) { fprintf(stderr, "%s\n", msg.c_str()); return 1; }
+  sc.tuning().launch.device_tail = !o.host_tail;
+  sc.tuning().launch.graph = o.graph;
+  if (o.hi > o.lo && sc.open_device(o.device, msg)) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
   if (o.hi > o.lo) fprintf(stderr, "[es_sieve] device %s\n", sc.device().device_name().c_str());
   const double t_ready = wall_s();
 

@@ -41,6 +41,30 @@ class EsTiming(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class EsServeConfig(ctypes.Structure):
+    _fields_ = [("slice", U64), ("batch_ints", U64), ("batch_windows", ctypes.c_uint32),
+                ("slots", ctypes.c_uint32), ("hbm_fraction", ctypes.c_double), ("hbm_budget", U64),
+                ("emit_survivors", ctypes.c_int)]
+
+
+class EsRequestInfo(ctypes.Structure):
+    _fields_ = [("arrival_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("batches", U64), ("reserved", U64), ("exit_code", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class EsServeStats(ctypes.Structure):
+    _fields_ = [(n, U64) for n in ("submitted", "completed", "refused", "batches", "slices", "ints", "budget",
+                                   "resident", "peak_reserved")] + \
+               [(n, ctypes.c_uint32) for n in ("max_active", "max_batch_requests", "max_queued")] + \
+               [("busy_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class EsError(RuntimeError):
     pass
 
@@ -84,6 +108,16 @@ def lib():
             "es_mark_count": (ctypes.c_int, [ctypes.c_void_p, U64, U64, U64, P(U64), P(U64), P(U64), P(U64)]),
             "es_lds_peak": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
             "es_wheel_classes": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_size_t)]),
+            "es_serve_config_default": (None, [P(EsServeConfig)]),
+            "es_server_open": (ctypes.c_int, [ctypes.c_void_p, P(EsServeConfig), P(ctypes.c_void_p)]),
+            "es_server_submit": (ctypes.c_int, [ctypes.c_void_p, U64, U64, P(U64)]),
+            "es_server_wait": (ctypes.c_int, [ctypes.c_void_p, U64, ctypes.c_double, P(ctypes.c_void_p),
+                                              P(ctypes.c_size_t), P(EsRequestInfo)]),
+            "es_server_stats": (ctypes.c_int, [ctypes.c_void_p, P(EsServeStats)]),
+            "es_server_close": (None, [ctypes.c_void_p]),
+            "es_serve_trace": (ctypes.c_int, [ctypes.c_void_p, P(EsServeConfig), P(ctypes.c_double), P(U64),
+                                              P(U64), ctypes.c_size_t, ctypes.c_int, P(ctypes.c_void_p),
+                                              P(ctypes.c_size_t), P(EsRequestInfo), P(EsServeStats)]),
             "es_free": (None, [ctypes.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -260,3 +294,77 @@ class Scanner:
         a, b, c, d = U64(), U64(), U64(), U64()
         _check(lib().es_wheel_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
         return {"lanes": a.value, "wheel_pmin": b.value, "nfilters": c.value, "ncov": d.value}
+
+
+def serve_config(**kw):
+    """es_serve_config with the library defaults, overridden by keyword."""
+    c = EsServeConfig()
+    lib().es_serve_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """A continuous-batching request server over a Scanner (es_server_*). While it is
+    open it owns the scanner."""
+
+    def __init__(self, scanner, **cfg):
+        self.h = ctypes.c_void_p()
+        _check(lib().es_server_open(scanner.h, ctypes.byref(serve_config(**cfg)), ctypes.byref(self.h)))
+
+    def submit(self, lo, hi):
+        i = U64()
+        _check(lib().es_server_submit(self.h, lo, hi, ctypes.byref(i)))
+        return i.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """(record, info dict), or None on timeout."""
+        text, n, info = ctypes.c_void_p(), ctypes.c_size_t(), EsRequestInfo()
+        rc = _check(lib().es_server_wait(self.h, rid, timeout_ms, ctypes.byref(text), ctypes.byref(n),
+                                         ctypes.byref(info)))
+        if rc == 1:
+            return None
+        try:
+            return ctypes.string_at(text, n.value).decode(), info.as_dict()
+        finally:
+            lib().es_free(text)
+
+    def stats(self):
+        st = EsServeStats()
+        _check(lib().es_server_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().es_server_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def serve_trace(scanner, requests, realtime=True, **cfg):
+    """Replays [(arrival_s, lo, hi)] through a server: (records, infos, stats)."""
+    n = len(requests)
+    arr = (ctypes.c_double * max(1, n))(*[r[0] for r in requests])
+    lo = _u64s([r[1] for r in requests])
+    hi = _u64s([r[2] for r in requests])
+    text = ctypes.c_void_p()
+    rl = (ctypes.c_size_t * max(1, n))()
+    info = (EsRequestInfo * max(1, n))()
+    st = EsServeStats()
+    _check(lib().es_serve_trace(scanner.h, ctypes.byref(serve_config(**cfg)), arr, lo, hi, n,
+                                1 if realtime else 0, ctypes.byref(text), rl, info, ctypes.byref(st)))
+    try:
+        raw = ctypes.string_at(text, sum(rl[i] for i in range(n)))
+    finally:
+        lib().es_free(text)
+    out, pos = [], 0
+    for i in range(n):
+        out.append(raw[pos:pos + rl[i]].decode())
+        pos += rl[i]
+    return out, [info[i].as_dict() for i in range(n)], st.as_dict()

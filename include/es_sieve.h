@@ -168,6 +168,71 @@ int es_lds_peak(es_scanner *s, double *ops_per_s);
  * ref3_wheel (sieve/cuda/ref_host.h:33). */
 int es_wheel_classes(const es_scanner *s, uint64_t *out, size_t cap, size_t *n);
 
+/* ---------------------------------------------------------------- serving
+ * A continuous-batching request server over one scanner. The reference has no request
+ * server; each request's answer is exactly rung_scan3's stdout for `rung_scan3 LO HI`
+ * with the scanner's flags (rung_scan3.cpp:1128-1162), alone or in any mix. Admission is
+ * FIFO, bounded by request slots and an HBM budget (hbm_fraction of device memory, minus
+ * the scanner's resident tables); every device batch takes the next slice of each
+ * admitted request, so requests that arrive while a batch runs join the next one. While
+ * a server is open it owns the scanner: make no other call on that scanner until
+ * es_server_close. */
+typedef struct es_server es_server;
+
+typedef struct es_serve_config {
+  uint64_t slice;            /* integers per slice of a request (default 2^32) */
+  uint64_t batch_ints;       /* integers per device batch (default 2^35, >= slice) */
+  uint32_t batch_windows;    /* slices per device batch (default 4096) */
+  uint32_t slots;            /* requests admitted at once (default 256) */
+  double hbm_fraction;       /* admission budget / device memory (default 0.8) */
+  uint64_t hbm_budget;       /* bytes; nonzero overrides hbm_fraction */
+  int emit_survivors;        /* 0 = --no-emit-survivors (default 1) */
+} es_serve_config;
+
+typedef struct es_request_info {
+  double arrival_ms, admit_ms, done_ms;   /* server clock: submitted, admitted, record done */
+  uint64_t batches;                       /* device batches the request took part in */
+  uint64_t reserved;                      /* HBM bytes it held while admitted */
+  int exit_code;                          /* 0, 2 (survivors) or 1 (refused) */
+} es_request_info;
+
+typedef struct es_serve_stats {
+  uint64_t submitted, completed, refused, batches, slices, ints;
+  uint64_t budget, resident, peak_reserved;
+  uint32_t max_active, max_batch_requests, max_queued;
+  double busy_ms;            /* time inside device batches */
+  double wall_ms;            /* es_serve_trace: first arrival to last completion */
+} es_serve_stats;
+
+void es_serve_config_default(es_serve_config *cfg);
+
+/* Starts a server on s (uploads every layout the tuner can pick, sizes the budget). */
+int es_server_open(es_scanner *s, const es_serve_config *cfg, es_server **out);
+
+/* Submits [lo, hi), arriving now; *id identifies it. A refused interval
+ * (rung_scan3.cpp:1057-1069) completes at once with exit code 1 and the refusal message
+ * as its record. */
+int es_server_submit(es_server *v, uint64_t lo, uint64_t hi, uint64_t *id);
+
+/* Waits up to timeout_ms (negative: no limit) for request id. Returns ES_OK with the
+ * record in *record (free with es_free; *len its length) and *info filled, 1 on timeout,
+ * or a negative status. A finished request is handed out once. */
+int es_server_wait(es_server *v, uint64_t id, double timeout_ms, char **record, size_t *len,
+                   es_request_info *info);
+
+int es_server_stats(es_server *v, es_serve_stats *st);
+
+/* Finishes every submitted request, then stops the server. */
+void es_server_close(es_server *v);
+
+/* Replays a request trace through a server: request i is submitted at arrival_s[i]
+ * seconds after the start (realtime = 1) or all at once (realtime = 0). Records are
+ * concatenated in request order into *text (free with es_free), rec_len[i] and info[i]
+ * per request, stats (optional) for the run. */
+int es_serve_trace(es_scanner *s, const es_serve_config *cfg, const double *arrival_s, const uint64_t *lo,
+                   const uint64_t *hi, size_t n, int realtime, char **text, size_t *rec_len,
+                   es_request_info *info, es_serve_stats *stats);
+
 /* Frees memory returned by this library. */
 void es_free(void *p);
 

@@ -16,8 +16,10 @@ What it writes:
   class_tables.sha256    sha256 of the reference's certified class tables
   c4_windows.txt         512 batched windows, numpy PCG64 seed 0
   c5_trace.txt           Poisson request trace, 200 req/s, numpy PCG64 seed 1
+  c4/records_w<M>.txt    per-window records of the 512 batched windows (--c4-only)
+  c5/records_first400.txt  records of the trace's first 400 requests (--c5-only)
 
-Usage: python3 tests/golden/make_golden.py [--skip-slow]
+Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only]
   --skip-slow  leave out the top-of-range interval, whose reference startup
                (a base-prime sieve to sqrt(hi)) takes minutes and ~9 GB.
 """
@@ -187,10 +189,39 @@ def c4_records():
         print("c4 wheel", M, "records", len(out), "cross-checked against the reference:", checked, flush=True)
 
 
+C5_GOLDEN = 400
+
+
+def c5_records():
+    """Records of the first C5_GOLDEN requests of the serving trace (wheel 2042040, the GPU
+    filter table), from the oracle; requests with hi < 1e15 are also run through the
+    reference binary and must agree byte for byte."""
+    rows = [l.split() for l in open(os.path.join(HERE, "c5_trace.txt")) if not l.startswith("#")]
+    d = os.path.join(HERE, "c5")
+    os.makedirs(d, exist_ok=True)
+    out, checked = [], 0
+    for _, lo, hi in rows[:C5_GOLDEN]:
+        argv = [lo, hi, "--wheel", "2042040", "--filters", F204]
+        rec = subprocess.run([ORACLE_BIN] + argv + ["--threads", "8"], stdout=subprocess.PIPE,
+                             stderr=subprocess.DEVNULL).stdout
+        if int(hi) < 10 ** 15:
+            ref = subprocess.run([REF_BIN] + argv, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL).stdout
+            if ref != rec:
+                raise SystemExit("oracle and reference disagree on request %s %s" % (lo, hi))
+            checked += 1
+        out.append(rec)
+    with open(os.path.join(d, "records_first%d.txt" % C5_GOLDEN), "wb") as f:
+        f.write(b"".join(out))
+    print("c5 records", len(out), "cross-checked against the reference:", checked, flush=True)
+
+
 def main():
     skip_slow = "--skip-slow" in sys.argv
     if "--c4-only" in sys.argv:
         c4_records()
+        return
+    if "--c5-only" in sys.argv:
+        c5_records()
         return
     os.makedirs(os.path.join(HERE, "records"), exist_ok=True)
     c4_windows()
