@@ -65,36 +65,75 @@ __device__ __forceinline__ u32 mod64(u64 x, u32 s, u32 mg, u32 c32) {
   return t >= s ? t - s : t;
 }
 
-// Clears map bytes i, i + s, i + 2s, ... below nb (four stores per bound check).
+typedef u8 __attribute__((address_space(3))) *LdsU8;
+
+// Clears map bytes i, i + s, i + 2s, ... below nb: four LDS byte addresses advanced by
+// one add each per step, then at most three more. i < 2^16 and s < 2^16, so nothing
+// wraps. The empty asm makes the four addresses opaque to loop strength reduction,
+// which would otherwise rebuild each as base + offset with an extra add per store.
 __device__ __forceinline__ void clear_run(u8 *map, u32 i, u32 s, u32 nb) {
-  for (; i + 3u * s < nb; i += 4u * s) {
-    map[i] = 0;
-    map[i + s] = 0;
-    map[i + 2u * s] = 0;
-    map[i + 3u * s] = 0;
+  const u32 base = (u32)(size_t)(LdsU8)map;
+  const u32 end = base + nb;
+  u32 a = base + i, b = a + s, c = b + s, d = c + s;
+  const u32 s4 = 4u * s;
+  while (d < end) {
+    *(LdsU8)(size_t)a = 0;
+    *(LdsU8)(size_t)b = 0;
+    *(LdsU8)(size_t)c = 0;
+    *(LdsU8)(size_t)d = 0;
+    a += s4;
+    b += s4;
+    c += s4;
+    d += s4;
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
   }
-  for (; i < nb; i += s) map[i] = 0;
+  if (a < end) *(LdsU8)(size_t)a = 0;
+  if (b < end) *(LdsU8)(size_t)b = 0;
+  if (c < end) *(LdsU8)(size_t)c = 0;
 }
 
+typedef const u32 __attribute__((address_space(1))) *GlobalU32;
+
 // Marks ordered filters [from, to), one filter per thread. Filters are sorted by
-// ascending stride, so 64 consecutive ones ("a bin") have similar hit counts; waves pull
-// bins from an LDS counter, heaviest first, which balances the workgroup without any
-// host planning.
+// ascending stride, so 64 consecutive ones ("a bin") have similar hit counts. Wave w
+// takes bins w, w + nwaves, ... (heaviest first, interleaved, so the waves end close
+// together with no shared counter), and loads the next bin's offset and constants
+// before marking the current one. kWide: the tile start k0 needs 64 bits; otherwise
+// its remainders are one 32-bit reduction.
+template <bool kWide>
 __device__ __forceinline__ void mark_range(u8 *map, u32 nb, const LayoutDev *L, const u16 *offrow, u64 k0,
-                                           u32 from, u32 to, u32 *bin_ctr, u32 lane64) {
+                                           u32 from, u32 to, u32 wave, u32 nwaves, u32 lane64) {
   const u32 nbins = (to - from + 63u) >> 6;
-  for (;;) {
-    u32 b = 0;
-    if (lane64 == 0) b = atomicAdd(bin_ctr, 1u);
-    b = __shfl(b, 0, 64);
-    if (b >= nbins) break;
-    const u32 j = from + (b << 6) + lane64;
-    if (j >= to) continue;
-    const u32 o = offrow[j];
-    if (o == 0xFFFFu) continue;
-    const uint4 f = L->fc[j];                       // {s, magic, 2^32 mod s}
-    const u32 km = mod64(k0, f.x, f.y, f.z);
-    clear_run(map, o >= km ? o - km : o + f.x - km, f.x, nb);
+  const u32 klo = (u32)k0;
+  const GlobalU32 fc = (GlobalU32)L->fc;    // per filter {s, floor(2^32/s), 2^32 mod s, 0}
+  u32 b = wave;
+  u32 j = from + (b << 6) + lane64;
+  u32 o = 0xFFFFu, fs = 1, fm = 0, fr = 0;
+  if (b < nbins && j < to) {
+    o = offrow[j];
+    fs = fc[4 * j];
+    fm = fc[4 * j + 1];
+    fr = fc[4 * j + 2];
+  }
+  while (b < nbins) {
+    const u32 bn = b + nwaves;
+    const u32 jn = from + (bn << 6) + lane64;
+    u32 on = 0xFFFFu, ns = 1, nm = 0, nr = 0;
+    if (bn < nbins && jn < to) {
+      on = offrow[jn];
+      ns = fc[4 * jn];
+      nm = fc[4 * jn + 1];
+      nr = fc[4 * jn + 2];
+    }
+    if (o != 0xFFFFu) {
+      const u32 km = kWide ? mod64(k0, fs, fm, fr) : mod32(klo, fs, fm);
+      clear_run(map, o >= km ? o - km : o + fs - km, fs, nb);
+    }
+    b = bn;
+    o = on;
+    fs = ns;
+    fm = nm;
+    fr = nr;
   }
 }
 
@@ -119,11 +158,15 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
 // lane, megabytes in all) do not stay in L2, so each tile streams its row into LDS with
 // coalesced 16-byte loads that overlap the mask init, instead of every bin waiting on
 // its own global load.
+//
+// Each tile writes its counters {positions, alive after covers, survivors} to its own
+// slot of tile_counts: no counter is shared between workgroups (the host sums them per
+// window), so only tiles with survivors touch a shared word (the survivor count).
 template <bool kPersistent>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
                                                 const LayoutDev *__restrict__ lays,
-                                                u64 *__restrict__ counts, u64 *__restrict__ surv_p,
+                                                uint4 *__restrict__ tile_counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
                                                 u32 *__restrict__ surv_count, u32 *__restrict__ queue,
                                                 u32 lds_words, u32 off_words) {
@@ -131,8 +174,8 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
   u16 *soff = (u16 *)(lds + lds_words);
-  u32 *sc = lds + lds_words + off_words;   // [0] tile, [1] base, [2] bin counter B, [3] bin counter B',
-                               // [16..) per-wave partial sums (3 x kMaxWaves)
+  u32 *sc = lds + lds_words + off_words;   // [0] tile, [1] survivor base,
+                                           // [16..) per-wave partial sums (3 x kMaxWaves)
   u32 *s_wpos = sc + 16, *s_waft = sc + 16 + kMaxWaves, *s_wsur = sc + 16 + 2 * kMaxWaves;
   const u32 tid = threadIdx.x, bs = blockDim.x;
   const u32 lane64 = tid & 63u, wave = tid >> 6, nwaves = (bs + 63u) >> 6;
@@ -150,7 +193,6 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
     const u32 P = L->P;
-    if (tid == 0) { sc[2] = 0; sc[3] = 0; }
     {
       const uint4 *src = (const uint4 *)(L->off + (size_t)td.lane * L->ostride);
       uint4 *dst = (uint4 *)soff;
@@ -177,13 +219,16 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     }
     __syncthreads();
     // ---- stage B: certified filters
-    mark_range(map, nb, L, offrow, td.k0, 0, L->ncov, sc + 2, lane64);
+    const bool wide = (td.k0 >> 32) != 0;
+    if (wide) mark_range<true>(map, nb, L, offrow, td.k0, 0, L->ncov, wave, nwaves, lane64);
+    else mark_range<false>(map, nb, L, offrow, td.k0, 0, L->ncov, wave, nwaves, lane64);
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
       for (u32 c = tid; c < nc; c += bs) my_after += __popc(chunk_bits(map32, c));
       // ---- stage B': small primes
-      mark_range(map, nb, L, offrow, td.k0, L->ncov, L->nf, sc + 3, lane64);
+      if (wide) mark_range<true>(map, nb, L, offrow, td.k0, L->ncov, L->nf, wave, nwaves, lane64);
+      else mark_range<false>(map, nb, L, offrow, td.k0, L->ncov, L->nf, wave, nwaves, lane64);
       __syncthreads();
     }
     u32 my_surv = 0;
@@ -208,10 +253,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     if (tid == 0) {
       u32 tot_p = 0, tot_a = 0;
       for (u32 i = 0; i < nwaves; ++i) { tot_p += s_wpos[i]; tot_a += s_waft[i]; }
-      u64 *c = counts + (size_t)td.win * 3;
-      if (tot_p) atomicAdd((unsigned long long *)&c[0], (unsigned long long)tot_p);
-      if (tot_a) atomicAdd((unsigned long long *)&c[1], (unsigned long long)tot_a);
-      if (tot_s) atomicAdd((unsigned long long *)&c[2], (unsigned long long)tot_s);
+      tile_counts[t] = make_uint4(tot_p, tot_a, tot_s, 0u);
       sc[1] = tot_s ? atomicAdd(surv_count, tot_s) : 0u;
     }
     __syncthreads();
@@ -236,11 +278,10 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   }
 }
 
-// Zeroes the batch's counters. Sizes come from the device header, so a captured graph
-// of reset + sieve + MR serves every batch that fits the buffers.
-__global__ void k_reset(const BatchHdr *__restrict__ hdr, u64 *counts, u32 *surv_count, u32 *queue) {
-  const u32 n = hdr->nwin * 3;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counts[i] = 0;
+// Zeroes the survivor count and the tile queue (a fill kernel rather than a memset, so
+// a captured graph holds no memset node). Every other per-batch output is written in
+// full by the kernels, so one captured graph serves every batch that fits the buffers.
+__global__ void k_reset(u32 *surv_count, u32 *queue) {
   if (blockIdx.x == 0 && threadIdx.x == 0) { *surv_count = 0; *queue = 0; }
 }
 
@@ -827,17 +868,17 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
 
 // Copies one batch's results into pinned host memory with plain stores over the bus
 // (zero-copy), so the host needs one stream synchronisation and no staging copies:
-// the survivor count, the window counters, and per survivor p, window, primality flag
+// the survivor count, the per-tile counters, and per survivor p, window, primality flag
 // and (primes only) the stage-D result.
-__global__ void k_pack(const BatchHdr *__restrict__ hdr, const u64 *__restrict__ counts,
+__global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict__ tile_counts,
                        const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
                        const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
-                       const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, u64 *h_counts,
+                       const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, uint4 *h_tc,
                        u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  const u32 nc = hdr->nwin * 3, n = min(*surv_count, cap);
+  const u32 nt = hdr->ntiles, n = min(*surv_count, cap);
   if (g == 0) h_count[0] = *surv_count;
-  for (u32 i = g; i < nc; i += stride) h_counts[i] = counts[i];
+  for (u32 i = g; i < nt; i += stride) h_tc[i] = tile_counts[i];
   for (u32 i = g; i < n; i += stride) {
     const u8 f = flags[i];
     h_p[i] = surv_p[i];
@@ -898,8 +939,7 @@ struct DeviceCtx::Impl {
   size_t tiles_cap = 0;
   dev::BatchHdr *d_hdr = nullptr;
   dev::BatchHdr *h_hdr = nullptr;
-  u64 *d_counts = nullptr;
-  size_t win_cap = 0;
+  uint4 *d_tile_counts = nullptr;   // per tile {positions, alive after covers, survivors, 0}
   u64 *d_surv_p = nullptr;
   u32 *d_surv_w = nullptr;
   uint8_t *d_flags = nullptr;
@@ -912,12 +952,14 @@ struct DeviceCtx::Impl {
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
   // writes the results into (device-side addresses in the d_h* twins).
   TileDesc *h_tiles = nullptr;
-  u64 *h_counts = nullptr, *h_surv_p = nullptr;
+  uint4 *h_tile_counts = nullptr;
+  u64 *h_surv_p = nullptr;
   u32 *h_surv_w = nullptr;
   uint8_t *h_flags = nullptr;
   dev::RungOut *h_rung = nullptr;
   u32 *dh_count = nullptr;
-  u64 *dh_counts = nullptr, *dh_surv_p = nullptr;
+  uint4 *dh_tile_counts = nullptr;
+  u64 *dh_surv_p = nullptr;
   u32 *dh_surv_w = nullptr;
   uint8_t *dh_flags = nullptr;
   dev::RungOut *dh_rung = nullptr;
@@ -936,12 +978,12 @@ DeviceCtx::~DeviceCtx() {
     if (m.stream) (void)hipStreamSynchronize(m.stream);
     for (DevLayoutBufs &b : m.layouts)
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
-    for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_counts,
+    for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_tile_counts,
                     (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one,
                     (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.qm, (void *)m.trial.inv,
                     (void *)m.trial.lim})
       if (p) (void)hipFree(p);
-    for (void *p : {(void *)m.h_hdr, (void *)m.h_count, (void *)m.h_tiles, (void *)m.h_counts,
+    for (void *p : {(void *)m.h_hdr, (void *)m.h_count, (void *)m.h_tiles, (void *)m.h_tile_counts,
                     (void *)m.h_surv_p, (void *)m.h_surv_w, (void *)m.h_flags, (void *)m.h_rung})
       if (p) (void)hipHostFree(p);
     if (m.graph_exec) (void)hipGraphExecDestroy(m.graph_exec);
@@ -1104,14 +1146,11 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     if (m.d_tiles) HIP_TRY(hipFree(m.d_tiles));
     m.tiles_cap = std::max<size_t>(nt, 2 * m.tiles_cap);
     HIP_TRY(hipMalloc(&m.d_tiles, m.tiles_cap * sizeof(TileDesc)));
-    if (pinned((void **)&m.h_tiles, nullptr, m.tiles_cap * sizeof(TileDesc))) return -1;
-    m.graph_ok = false;
-  }
-  if (nwin > m.win_cap) {
-    if (m.d_counts) HIP_TRY(hipFree(m.d_counts));
-    m.win_cap = std::max<size_t>(nwin, 2 * m.win_cap);
-    HIP_TRY(hipMalloc(&m.d_counts, m.win_cap * 3 * sizeof(u64)));
-    if (pinned((void **)&m.h_counts, (void **)&m.dh_counts, m.win_cap * 3 * sizeof(u64))) return -1;
+    if (m.d_tile_counts) HIP_TRY(hipFree(m.d_tile_counts));
+    HIP_TRY(hipMalloc(&m.d_tile_counts, m.tiles_cap * sizeof(uint4)));
+    if (pinned((void **)&m.h_tiles, nullptr, m.tiles_cap * sizeof(TileDesc)) ||
+        pinned((void **)&m.h_tile_counts, (void **)&m.dh_tile_counts, m.tiles_cap * sizeof(uint4)))
+      return -1;
     m.graph_ok = false;
   }
   auto grow_surv = [&](size_t need) -> int {
@@ -1175,15 +1214,15 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   // The device part of one batch: reset, sieve, Miller-Rabin. The persistent grid does
   // not depend on the batch, which is what lets a graph of it be replayed.
   auto enqueue = [&](bool timed) -> int {
-    dev::k_reset<<<64, 256, 0, m.stream>>>(m.d_hdr, m.d_counts, m.d_scalars, m.d_scalars + 1);
+    dev::k_reset<<<1, 64, 0, m.stream>>>(m.d_scalars, m.d_scalars + 1);
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(m.ev1, m.stream));
     if (cfg.persistent) {
-      dev::k_sieve<true><<<pgrid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
+      dev::k_sieve<true><<<pgrid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_tile_counts,
                                                               m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
                                                               m.d_scalars, m.d_scalars + 1, lds_words, off_words);
     } else {
-      dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_counts,
+      dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_tile_counts,
                                                                  m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
                                                                  m.d_scalars, m.d_scalars + 1, lds_words, off_words);
     }
@@ -1200,9 +1239,9 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
                                                                m.d_flags, m.trial, m.d_rung);
       HIP_TRY(hipGetLastError());
     }
-    dev::k_pack<<<std::max(1, cus_), 256, 0, m.stream>>>(m.d_hdr, m.d_counts, m.d_scalars, (u32)m.surv_cap,
+    dev::k_pack<<<std::max(1, cus_), 256, 0, m.stream>>>(m.d_hdr, m.d_tile_counts, m.d_scalars, (u32)m.surv_cap,
                                                          m.d_surv_p, m.d_surv_w, m.d_flags, m.d_rung,
-                                                         cfg.device_tail ? 1u : 0u, m.dh_count, m.dh_counts,
+                                                         cfg.device_tail ? 1u : 0u, m.dh_count, m.dh_tile_counts,
                                                          m.dh_surv_p, m.dh_surv_w, m.dh_flags, m.dh_rung);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1247,7 +1286,6 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   }
   // Everything is in the pinned mirrors now (k_pack ran before the synchronisation).
   const u32 cnt = *m.h_count;
-  const u64 *raw = m.h_counts;
   out.surv_p.assign(m.h_surv_p, m.h_surv_p + cnt);
   out.surv_w.assign(m.h_surv_w, m.h_surv_w + cnt);
   out.surv_prime.resize(cnt);
@@ -1258,7 +1296,12 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     for (u32 i = 0; i < cnt; ++i)
       if (out.surv_prime[i]) std::memcpy(&out.rung[i], &m.h_rung[i], sizeof(RungResult));
   }
-  for (u32 w = 0; w < nwin; ++w) out.counts[w] = WinCounts{raw[w * 3], raw[w * 3 + 1], raw[w * 3 + 2]};
+  for (size_t i = 0; i < nt; ++i) {
+    WinCounts &c = out.counts[tiles[i].win];
+    c.positions += m.h_tile_counts[i].x;
+    c.after_covers += m.h_tile_counts[i].y;
+    c.survivors += m.h_tile_counts[i].z;
+  }
   return 0;
 }
 
@@ -1300,7 +1343,7 @@ size_t DeviceCtx::resident_bytes() const {
   const Impl &m = *impl_;
   const size_t per_surv = sizeof(u64) * 2 + sizeof(u32) + 1 + sizeof(dev::RungOut);
   return layout_bytes_ + m.trial.n * (sizeof(uint4) + 16) + m.tiles_cap * sizeof(TileDesc) +
-         m.win_cap * 3 * sizeof(u64) + m.surv_cap * per_surv;
+         m.tiles_cap * sizeof(uint4) + m.surv_cap * per_surv;
 }
 
 int DeviceCtx::lds_peak(double &ops_per_s) {
