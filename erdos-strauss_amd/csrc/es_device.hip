@@ -520,10 +520,10 @@ __device__ __forceinline__ u64 inv64(u64 n) {
 
 __device__ __forceinline__ u64 neg_inv(u64 n) { return 0 - inv64(n); }
 
-// a * 2^64 mod n for a < 2^8, by double-and-add from one = 2^64 mod n.
+// a * 2^64 mod n for a < 2^16, by double-and-add from one = 2^64 mod n.
 __device__ __forceinline__ u64 small_to_mont(u32 a, u64 one, u64 n) {
   u64 am = 0;
-  for (int bit = 7; bit >= 0; --bit) {
+  for (int bit = 15; bit >= 0; --bit) {
     am = add_mod(am, am, n);
     if ((a >> bit) & 1) am = add_mod(am, one, n);
   }
@@ -550,13 +550,55 @@ __device__ bool wave_is_prime(u64 n, u32 lane) {
 
 constexpr u64 kRhoFail = 0, kRhoStop = 1;
 
-// Brent's rho with an independent walk per lane (x -> x^2 + c, c = lane + 1, Montgomery
-// form). Returns a nontrivial factor of the odd composite non-square n, kRhoFail when
-// every walk fails inside the budget (the caller then hands the prime to the host), or
-// kRhoStop once *stop drops below k (a smaller rung is settled; the result is moot).
-__device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
+// Block-wide help with one rho at a time. The wave that needs a cofactor split (the
+// owner) publishes n; waves done with their rungs run walks of their own on it and post
+// the first factor they find, which the owner checks and takes.
+struct RhoShare {
+  u64 n, factor;
+  u32 gen, busy, done, pad;
+};
+
+// How one rho call takes part: solo (rs null), owner of the published split, or helper
+// of split `gen`. Walk constants start above cbase, so no two waves repeat a walk.
+struct RhoRole {
+  RhoShare *rs;
+  u32 helper, gen, cbase;
+};
+
+// The per-batch exchange. Owner: a posted factor that divides n, else 0 (a stale post
+// from an earlier split is cleared). Helper: kRhoStop once its split is over.
+__device__ __forceinline__ u64 rho_coop(const RhoRole &ro, u64 n) {
+  if (!ro.rs) return 0;
+  volatile RhoShare *v = ro.rs;
+  if (ro.helper)
+    return (__builtin_amdgcn_readfirstlane(v->busy) == 0 || __builtin_amdgcn_readfirstlane(v->gen) != ro.gen)
+               ? kRhoStop
+               : 0;
+  const u64 g = __builtin_amdgcn_readfirstlane((u32)v->factor) |
+                ((u64)__builtin_amdgcn_readfirstlane((u32)(v->factor >> 32)) << 32);
+  if (g == 0) return 0;
+  if (g > 1 && g < n && n % g == 0) return g;
+  atomicCAS((unsigned long long *)&ro.rs->factor, (unsigned long long)g, 0ull);
+  return 0;
+}
+
+// What a walk that found factor g returns: the owner or a solo wave keeps it, a helper
+// posts it (first post wins) and stops.
+__device__ __forceinline__ u64 rho_found(const RhoRole &ro, u64 g, u32 lane) {
+  if (!ro.helper) return g;
+  if (lane == 0 && ro.rs->gen == ro.gen)
+    atomicCAS((unsigned long long *)&ro.rs->factor, 0ull, (unsigned long long)g);
+  return kRhoStop;
+}
+
+// Brent's rho with an independent walk per lane (x -> x^2 + c, c = cbase + lane + 1,
+// Montgomery form). Returns a nontrivial factor of the odd composite non-square n,
+// kRhoFail when every walk fails inside the budget (the caller then hands the prime to
+// the host), or kRhoStop once *stop drops below k (a smaller rung is settled; the result
+// is moot) or, for a helper, once its split is over.
+__device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const RhoRole &ro) {
   const u64 ninv = neg_inv(n), one = (0 - n) % n;
-  const u64 c = small_to_mont(lane + 1, one, n);
+  const u64 c = small_to_mont(ro.cbase + lane + 1, one, n);
   auto f = [&](u64 v) { return add_mod(mont_mul(v, v, n, ninv), c, n); };
   u64 y = small_to_mont(2, one, n), x = y, ys = y, acc = one;
   bool alive = true;
@@ -582,9 +624,11 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
         if (g == 1) alive = false;
       }
       const u64 found = __ballot(g != 1);
-      if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
+      if (found) return rho_found(ro, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
       if (__ballot(alive) == 0) return kRhoFail;
-      if (*stop < k) return kRhoStop;
+      if (stop && *stop < k) return kRhoStop;
+      const u64 h = rho_coop(ro, n);
+      if (h) return h;
     }
   }
   return kRhoFail;
@@ -605,15 +649,15 @@ __device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, doub
 constexpr int kRhoWalks = 2;   // independent walks per lane (interleaved for ILP)
 
 // wave_rho for n < 2^50 (same contract): 64 x kRhoWalks walks x -> x^2 + c with
-// c = lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts to
-// integers, and gcd(a, n) does not depend on which representative of a mod n it sees.
-__device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
+// c = cbase + lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts
+// to integers, and gcd(a, n) does not depend on which representative of a mod n it sees.
+__device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, const RhoRole &ro) {
   const double nd = (double)n, ninv = 1.0 / nd;
   double c[kRhoWalks], y[kRhoWalks], x[kRhoWalks], ys[kRhoWalks], acc[kRhoWalks];
   bool alive[kRhoWalks];
 #pragma unroll
   for (int w = 0; w < kRhoWalks; ++w) {
-    c[w] = (double)(lane + 1 + 64 * w);
+    c[w] = (double)(ro.cbase + lane + 1 + 64 * w);
     y[w] = x[w] = ys[w] = 2;
     acc[w] = 1;
     alive[w] = true;
@@ -660,9 +704,11 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k) {
         any |= alive[w];
       }
       const u64 found = __ballot(g != 1);
-      if (found) return __shfl(g, __ffsll((long long)found) - 1, 64);
+      if (found) return rho_found(ro, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
       if (__ballot(any) == 0) return kRhoFail;
-      if (*stop < k) return kRhoStop;
+      if (stop && *stop < k) return kRhoStop;
+      const u64 h = rho_coop(ro, n);
+      if (h) return h;
     }
   }
   return kRhoFail;
@@ -700,7 +746,7 @@ constexpr int kFacOk = 0, kFacHost = 1, kFacStop = 2;
 // primes below 2^16 that divide it. kFacHost if the device declines, kFacStop if a
 // smaller rung settled meanwhile.
 __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &T, WaveFac &F, u32 lane,
-                           const volatile u32 *stop, u32 k) {
+                           const volatile u32 *stop, u32 k, RhoShare *rs) {
   if (lane == 0) { F.k = 0; F.bad = 0; }
   wave_sync();
   u64 n = a;
@@ -735,7 +781,20 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
       if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
       const u64 s = isqrt_dev(m);
       if (s * s == m) { fac_add(F, s, 2, lane); continue; }
-      const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k) : wave_rho(m, lane, stop, k);
+      // Become the owner of the block's shared split if nobody holds it.
+      u32 own = 0;
+      if (lane == 0) own = atomicCAS(&rs->busy, 0u, 1u) == 0u ? 1u : 0u;
+      own = __shfl(own, 0, 64);
+      if (own && lane == 0) {
+        rs->n = m;
+        rs->factor = 0;
+        __threadfence_block();
+        atomicAdd(&rs->gen, 1u);
+      }
+      wave_sync();
+      const RhoRole ro{own ? rs : nullptr, 0u, 0u, 0u};
+      const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k, ro) : wave_rho(m, lane, stop, k, ro);
+      if (own && lane == 0) atomicExch(&rs->busy, 0u);
       if (d == kRhoStop) return kFacStop;
       if (d == kRhoFail || top + 2 > 4) return kFacHost;
       stack[top++] = d;
@@ -798,6 +857,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   __shared__ u32 nfac[kRungSlots];
   __shared__ u32 next_k, found_k, host_k, stop_k;
   __shared__ RungOut wres[kRungWaves];
+  __shared__ RhoShare rs;
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
@@ -810,7 +870,11 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     const u64 A0 = (p + r0) / 4;
     if (tid < kRungSlots) nfac[tid] = 0;
     if (tid < kRungWaves) wres[tid] = RungOut{0, 0, 0, 0, kRungNone};
-    if (tid == 0) { next_k = 0; found_k = host_k = stop_k = kRungSlots; }
+    if (tid == 0) {
+      next_k = 0;
+      found_k = host_k = stop_k = kRungSlots;
+      rs = RhoShare{0, 0, 0, 0, 0, 0};
+    }
     __syncthreads();
     // Sieve the 64 consecutive A_j by every odd prime below 2^16.
     for (u32 i = tid; i < T.n; i += blockDim.x) {
@@ -835,7 +899,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
       if (r > 1 && ga != 1) continue;
       const u32 ns = nfac[k];
-      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k);
+      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k, &rs);
       if (fr == kFacStop) break;
       if (fr == kFacHost) {
         if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
@@ -849,6 +913,28 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
           atomicMin(&stop_k, k);
         }
         break;
+      }
+    }
+    // A wave done with its rungs helps whichever wave still splits a cofactor; all leave
+    // once every wave is done with its rungs (each rung loop ends: its rho is bounded).
+    if (lane == 0) atomicAdd(&rs.done, 1u);
+    {
+      volatile RhoShare *v = &rs;
+      u32 seen = 0;
+      for (;;) {
+        if (__builtin_amdgcn_readfirstlane(v->done) >= kRungWaves) break;
+        const u32 gen = __builtin_amdgcn_readfirstlane(v->gen);
+        if (__builtin_amdgcn_readfirstlane(v->busy) && gen != seen) {
+          seen = gen;
+          const u64 n = __builtin_amdgcn_readfirstlane((u32)v->n) |
+                        ((u64)__builtin_amdgcn_readfirstlane((u32)(v->n >> 32)) << 32);
+          if (__builtin_amdgcn_readfirstlane(v->gen) == gen && n > 1) {
+            const RhoRole ro{&rs, 1u, gen, 128u * (wv + 1)};
+            (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
+          }
+        } else {
+          __builtin_amdgcn_s_sleep(2);
+        }
       }
     }
     __syncthreads();
