@@ -180,6 +180,13 @@ int es_set_device_tail(es_scanner *s, int on) {
   return ES_OK;
 }
 
+int es_set_order(es_scanner *s, int order, uint64_t seg) {
+  if (!s || (order != 0 && order != 1)) return fail(ES_ERR_ARG, "order must be 0 (lane) or 1 (round)");
+  s->sc.order = order ? es::Order::Round : es::Order::Lane;
+  if (seg) s->sc.round_seg = seg;
+  return ES_OK;
+}
+
 int es_set_layout(es_scanner *s, uint64_t Mp) {
   if (!s) return fail(ES_ERR_ARG, "null scanner");
   if (Mp && s->sc.plan().M % Mp) return fail(ES_ERR_ARG, "layout modulus must divide the wheel modulus");

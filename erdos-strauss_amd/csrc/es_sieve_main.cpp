@@ -383,8 +383,6 @@ int main(int argc, char **argv) {
   struct Line { u64 k1, k2, p; std::string text; };
   std::vector<Line> all;
   ScanTiming acc;
-  const u64 M = o.M;
-  const std::vector<u64> &cls = pl.wheel.classes;
   for (u64 a = o.lo; a < o.hi;) {
     const u64 b = o.hi - a > o.chunk ? a + o.chunk : o.hi;
     std::vector<WindowRecord> recs;
@@ -395,14 +393,8 @@ int main(int argc, char **argv) {
     total.composite += r.composite; total.rung += r.rung; total.direct += r.direct;
     total.survivors += r.survivors;
     for (size_t i = 0; i < r.lines.size(); ++i) {
-      const u64 p = r.line_p[i], rho = p % M;
-      const u64 rank = (u64)(std::lower_bound(cls.begin(), cls.end(), rho) - cls.begin());
-      Line L{rank, 0, p, r.lines[i]};
-      if (o.order == Order::Round) {
-        const u64 first = o.lo + (rho + M - o.lo % M) % M;
-        L.k1 = ((p - first) / M) / o.seg;
-        L.k2 = rank;
-      }
+      Line L{0, 0, r.line_p[i], r.lines[i]};
+      sc.emit_key(L.p, o.lo, L.k1, L.k2);   // the key of the whole interval, not the chunk
       all.push_back(std::move(L));
     }
     acc.plan_ms += tm.plan_ms; acc.batch_ms += tm.batch_ms; acc.device_ms += tm.device_ms;

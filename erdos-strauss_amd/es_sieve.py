@@ -99,6 +99,7 @@ def lib():
             "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "es_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
             "es_set_device_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+            "es_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, U64]),
             "es_certify_batch": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_void_p),
                                                 P(ctypes.c_size_t)]),
             "es_choose_layout": (U64, [ctypes.c_void_p, U64]),
@@ -236,6 +237,10 @@ class Scanner:
 
     def set_device_tail(self, on=True):
         _check(lib().es_set_device_tail(self.h, 1 if on else 0))
+
+    def set_order(self, order="lane", seg=0):
+        """Emission order of certificate lines: "lane" (rung_scan3) or "round" (sieve_cuda)."""
+        _check(lib().es_set_order(self.h, {"lane": 0, "round": 1}[order], seg))
 
     def certify_batch(self, primes):
         """Stage D on the device for a list of primes: (lines, how many the host finished)."""

@@ -113,6 +113,13 @@ int es_certify_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, si
 /* Stage D placement for the following scans: 1 = on the device (default), 0 = host. */
 int es_set_device_tail(es_scanner *s, int on);
 
+/* Emission order of certificate lines in the following records: 0 = lane-major
+ * (rung_scan3: rank of p mod M in the wheel, then p; rung_scan3.cpp:1146-1151), 1 =
+ * round-major (sieve_cuda: floor(J / seg), rank, p, with J = (p - first lane position)
+ * / M and seg positions per lane per round; sieve/cuda/scan.cu:295-309). seg 0 keeps
+ * the current value (default 262144, rung_scan3's --seg default). */
+int es_set_order(es_scanner *s, int order, uint64_t seg);
+
 /* Sub-wheel layout for the following scans: Mp must divide the wheel modulus; 0
  * restores the autotuner. No counterpart in the reference (whose only layout is
  * Mp = M, rung_scan3.cpp:711-716); output does not depend on it. */
