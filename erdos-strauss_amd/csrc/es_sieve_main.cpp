@@ -376,13 +376,33 @@ int main(int argc, char **argv) {
   const double t_ready = wall_s();
 
   // Chunks of at most o.chunk integers; counters add, lines are merged by the emission
-  // key of the whole interval.
+  // key of the whole interval. In round order the integers [lo + r*seg*M, lo +
+  // (r+1)*seg*M) hold exactly round r's positions of every lane, so with chunks cut at
+  // those boundaries no later chunk has a smaller key: each chunk's lines are sorted and
+  // written at once and memory stays bounded by one chunk. Lane order interleaves every
+  // chunk in every lane, so its lines are held to the end (as rung_scan3 holds them).
   WindowRecord total;
   total.lo = o.lo;
   total.hi = o.hi;
   struct Line { u64 k1, k2, p; std::string text; };
   std::vector<Line> all;
   ScanTiming acc;
+  const bool stream = o.order == Order::Round;
+  if (stream) {
+    const u128 round_w = (u128)o.seg * o.M;
+    const u128 want = (u128)o.chunk < round_w ? round_w : ((u128)o.chunk / round_w) * round_w;
+    o.chunk = want > (u128)~0ull ? ~0ull : (u64)want;
+  }
+  auto flush_lines = [&]() {
+    std::sort(all.begin(), all.end(), [](const Line &x, const Line &y) {
+      if (x.k1 != y.k1) return x.k1 < y.k1;
+      if (x.k2 != y.k2) return x.k2 < y.k2;
+      return x.p < y.p;
+    });
+    for (const Line &l : all)
+      if (o.emit_surv || l.text.compare(0, 9, "SURVIVOR ") != 0) fwrite(l.text.data(), 1, l.text.size(), stdout);
+    all.clear();
+  };
   for (u64 a = o.lo; a < o.hi;) {
     const u64 b = o.hi - a > o.chunk ? a + o.chunk : o.hi;
     std::vector<WindowRecord> recs;
@@ -404,14 +424,10 @@ int main(int argc, char **argv) {
       fprintf(stderr, "[es_sieve] chunk [%" PRIu64 ",%" PRIu64 ") layout=%" PRIu64 " tiles=%" PRIu64
               " survivors=%" PRIu64 " primes=%" PRIu64 "\n", a, b, r.layout_Mp, tm.tiles, tm.survivors, tm.primes);
     a = b;
+    if (stream) flush_lines();
   }
-  std::sort(all.begin(), all.end(), [](const Line &x, const Line &y) {
-    if (x.k1 != y.k1) return x.k1 < y.k1;
-    if (x.k2 != y.k2) return x.k2 < y.k2;
-    return x.p < y.p;
-  });
-  for (Line &l : all) total.lines.push_back(std::move(l.text));
-  const std::string text = sc.record_text(total, o.emit_surv);
+  flush_lines();
+  const std::string text = sc.record_text(total, o.emit_surv);   // no lines left: the SUMMARY
   fwrite(text.data(), 1, text.size(), stdout);
   fflush(stdout);
   const double t_end = wall_s();

@@ -18,8 +18,9 @@ What it writes:
   c5_trace.txt           Poisson request trace, 200 req/s, numpy PCG64 seed 1
   c4/records_w<M>.txt    per-window records of the 512 batched windows (--c4-only)
   c5/records_first400.txt  records of the trace's first 400 requests (--c5-only)
+  shards/lane/shard_*.meta shard metas of a 7-shard run of the reference binary (--shards-only)
 
-Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only]
+Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only | --shards-only]
   --skip-slow  leave out the top-of-range interval, whose reference startup
                (a base-prime sieve to sqrt(hi)) takes minutes and ~9 GB.
 """
@@ -215,8 +216,29 @@ def c5_records():
     print("c5 records", len(out), "cross-checked against the reference:", checked, flush=True)
 
 
+SHARD_LO, SHARD_HI, SHARD_N = 10 ** 12 + 7, 10 ** 12 + 4 * 10 ** 10 + 3, 7
+
+
+def shard_metas():
+    """Shard metas of [SHARD_LO, SHARD_HI) in SHARD_N shards, written by the production
+    driver (erdos-strauss_amd/run_shards.py) running the reference binary itself, in its
+    own (lane) order: the deterministic fields (certs, cert_sha256, SURVIVOR, SUMMARY)
+    are what a run of the drop-in must reproduce."""
+    out = os.path.join(HERE, "shards", "lane")
+    shutil.rmtree(out, ignore_errors=True)
+    subprocess.run([sys.executable, os.path.join(REPO, "erdos-strauss_amd", "run_shards.py"), str(SHARD_LO),
+                    str(SHARD_HI), str(SHARD_N), out, "--bin", REF_BIN, "--table", F204, "--order", "lane",
+                    "--no-gpu-check", "--threads", "8", "--retry-sleep", "0"], check=True)
+    for f in os.listdir(out):
+        if not f.endswith(".meta"):
+            os.remove(os.path.join(out, f))
+
+
 def main():
     skip_slow = "--skip-slow" in sys.argv
+    if "--shards-only" in sys.argv:
+        shard_metas()
+        return
     if "--c4-only" in sys.argv:
         c4_records()
         return
