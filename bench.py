@@ -50,6 +50,18 @@ WINDOW = 2 ** 22
 NOMINAL_LDS_BPS = 256 * 128 * 2.4e9        # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
 
 
+def profiled_traffic():
+    """HBM bytes per k_sieve launch on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if it is absent."""
+    path = os.path.join(REPO, "profiles", "k_sieve_c2_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def pct(vals, q):
     v = sorted(vals)
     k = min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))
@@ -288,7 +300,10 @@ def main():
                      "peak_measured": lds_ops * 8 / 1e9,
                      "frac_of_measured": achieved / (lds_ops * 8 / 1e9),
                      "marks": work["marks"], "bits": work["bits"], "pairs": work["pairs"],
-                     "kernel_ms": kernel_s * 1e3, "traffic": None},
+                     "algorithmic_bytes": lds_bytes, "kernel_ms": kernel_s * 1e3,
+                     "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/k_sieve_c2_hbm_traffic.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                       "per k_sieve dispatch on this workload, scaled by a 1 GiB copy"},
     }
     if not args.no_c4:
         out["c4"] = c4_bench(es)

@@ -192,3 +192,29 @@ def test_planted_mistake_is_caught():
     t[2] = str(int(t[2]) + 1)                     # a wrong A
     _, _, bad, _, _ = vc.verify_lines([" ".join(t)])
     assert bad == 1
+
+
+def test_exact_verifier_accepts_every_published_certificate():
+    """Every RUNG line of the reference's published records (decades d6..d19, the 5e18
+    window, the top-of-range edges) and every SOLVED row of its --check-direct battery
+    holds as an exact rational identity, with p prime and p == 1 (mod 24) -- the
+    build's own verifier against the reference's outputs."""
+    import glob
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("verify_certs", os.path.join(paths.PKG, "verify_certs.py"))
+    vc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(vc)
+    files = sorted(glob.glob(os.path.join(paths.GOLDEN, "rederive", "d*.txt")))
+    files += [os.path.join(paths.GOLDEN, "rederive", f) for f in
+              ("w19_certificates.txt", "w64_head.txt", "w64_tail.txt", "s0_head.txt", "s69_tail.txt")]
+    total = 0
+    for f in files:
+        nr, ns, bad, surv, problems = vc.verify_lines(open(f).read().splitlines(), lo=0, hi=1 << 64)
+        assert bad == 0 and surv == 0, (f, problems[:3])
+        total += nr
+    solved = 0
+    for f in glob.glob(os.path.join(paths.GOLDEN, "direct", "*.txt")):
+        nr, ns, bad, surv, problems = vc.verify_lines(open(f).read().splitlines())
+        assert bad == 0, (f, problems[:3])
+        solved += ns
+    assert total > 17000 and solved > 100
