@@ -637,8 +637,9 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
 
 // a * b mod n on FP64 values holding integers, n < 2^50, with signed residues:
 // q = rint(a*b/n) makes r = a*b - q*n exact (FMA residual, one exact FMA) with
-// |r| <= (1/2 + 3*2^-53*|a*b|/n) * n. So |a|, |b| < n + 128 give |r| < 7n/8 (the walk
-// needs no correction), and |a| < n, |b| < 2n + 256 give |r| < 5n/4 (one correction).
+// |r| <= (1/2 + 3*2^-53*|a*b|/n) * n. So |a|, |b| < n + 1024 give |r| < 7n/8 (the walk
+// stays below n with no correction), and |a| <= 2n, |b| < 2n give |r| < 2n (the product
+// of differences stays below 2n with no correction either).
 __device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, double ninv) {
   const double h = a * b;
   const double l = __fma_rn(a, b, -h);
@@ -663,10 +664,6 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
     alive[w] = true;
   }
   auto f = [&](double v, double cc) { return mulmod_sgn(v, v, nd, ninv) + cc; };
-  auto red = [&](double r) {
-    r = r >= nd ? r - nd : r;
-    return r <= -nd ? r + nd : r;
-  };
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
 #pragma unroll
     for (int w = 0; w < kRhoWalks; ++w) x[w] = y[w];
@@ -682,7 +679,7 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
 #pragma unroll
         for (int w = 0; w < kRhoWalks; ++w) {
           y[w] = f(y[w], c[w]);
-          acc[w] = red(mulmod_sgn(acc[w], x[w] - y[w], nd, ninv));
+          acc[w] = mulmod_sgn(acc[w], x[w] - y[w], nd, ninv);   // |acc| < 2n
         }
       }
       u64 g = 1;

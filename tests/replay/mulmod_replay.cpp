@@ -4,8 +4,9 @@
 //                quotient estimate, one exact FMA for h - q*n, one correction each way;
 //   mulmod_sgn   the signed-residue form rho walks in: a * b with q = rint(a*b/n) gives
 //                r == a*b (mod n) with |r| <= (1/2 + 3*2^-53*|a*b|/n) * n, exactly, for
-//                |a*b| < 2^101 -- so |r| < n for |a|, |b| < n + 64 (the walk) and
-//                |r| < 1.25 n for |a| < n, |b| < 2n + 128 (the product of differences).
+//                |a*b| < 2^102 -- so |r| < n for |a|, |b| < n + 64 (the walk), |r| < 1.25 n
+//                for |a| < n, |b| < 2n + 128, and |r| < 2n for |a| <= 2n, |b| < 2n (the
+//                accumulated product of differences, which therefore needs no correction).
 // IEEE-754 binary64 with a fused multiply-add is what the device executes, so std::fma
 // and std::rint (round to nearest even) give the same results bit for bit.
 //
@@ -100,6 +101,9 @@ int main(int argc, char **argv) {
     for (long long a : {1LL, m - 1, 1 - m, m / 3})
       for (long long b : {2 * m + 127, -2 * m - 127, 2 * m - 1, m + 1})
         if (check_sgn(a, b, n, 1.25)) return 1;
+    for (long long a : {2 * m, -2 * m, 2 * m - 1, 1 - 2 * m, m, 1LL})
+      for (long long b : {2 * m - 1, 1 - 2 * m, 2 * m - 3, m + 1, -m - 1})
+        if (check_sgn(a, b, n, 2.0)) return 1;
   }
   for (u64 t = 0; t < trials; ++t) {
     const int bits = 33 + (int)(next() % 18);
@@ -114,6 +118,9 @@ int main(int argc, char **argv) {
       const long long x = (long long)(next() % (2 * n - 1)) - m + 1;
       const long long d = (long long)(next() % (4 * n + 255)) - 2 * m - 127;
       if (check_sgn(x, d, n, 1.25)) return 1;
+      const long long acc = (long long)(next() % (4 * n + 1)) - 2 * m;       // |acc| <= 2n
+      const long long e = (long long)(next() % (4 * n - 1)) - 2 * m + 1;     // |e| < 2n
+      if (check_sgn(acc, e, n, 2.0)) return 1;
     }
   }
   printf("ok %llu\n", (unsigned long long)trials);
