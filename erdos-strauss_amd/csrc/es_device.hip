@@ -817,7 +817,8 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
     const u32 pm = small_mod(pr, r, mg, c32);
     // pm is a unit mod r on every rung that is not skipped: find its inverse.
     u32 pinv = 0;
-    for (u32 x = lane; x < r; x += 64) if ((pm * x) % r == 1 % r) pinv = x + 1;
+    const u32 one = r == 1 ? 0u : 1u;
+    for (u32 x = lane; x < r; x += 64) if (mod32(pm * x, r, mg) == one) pinv = x + 1;
     const u64 bal = __ballot(pinv != 0);
     pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
     for (u32 t = lane; t < r; t += 64) {
@@ -831,7 +832,7 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
         u128 npw;
         if (!mul_le(pw, (u128)pr, q, npw)) break;
         pw = npw;
-        s = (s * pinv) % r;
+        s = mod32(s * pinv, r, mg);   // < 255 * 255, and mg = floor(2^32 / r)
       }
       nxt[t] = acc;
     }
@@ -873,13 +874,25 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       rs = RhoShare{0, 0, 0, 0, 0, 0};
     }
     __syncthreads();
-    // Sieve the 64 consecutive A_j by every odd prime below 2^16.
-    for (u32 i = tid; i < T.n; i += blockDim.x) {
-      const uint4 c = T.qm[i];
-      const u32 x = mod64(A0, c.x, c.y, c.z);
-      for (u32 j = x ? c.x - x : 0; j < nj; j += c.x) {
-        const u32 s = atomicAdd(&nfac[j], 1u);
-        if (s < kSmallFac) fac[j][s] = (u16)i;
+    // Sieve the 64 consecutive A_j by every odd prime below 2^16 (four table loads in
+    // flight per thread before any is used: the loop is otherwise one load latency per
+    // prime).
+    for (u32 i0 = tid; i0 < T.n; i0 += 4 * blockDim.x) {
+      uint4 c[4];
+#pragma unroll
+      for (u32 u = 0; u < 4; ++u) {
+        const u32 i = i0 + u * blockDim.x;
+        c[u] = i < T.n ? T.qm[i] : make_uint4(1u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (u32 u = 0; u < 4; ++u) {
+        const u32 i = i0 + u * blockDim.x;
+        if (i >= T.n) continue;
+        const u32 x = mod64(A0, c[u].x, c[u].y, c[u].z);
+        for (u32 j = x ? c[u].x - x : 0; j < nj; j += c[u].x) {
+          const u32 s = atomicAdd(&nfac[j], 1u);
+          if (s < kSmallFac) fac[j][s] = (u16)i;
+        }
       }
     }
     __syncthreads();
