@@ -97,6 +97,18 @@ inline void lane_span(u64 lo, u64 hi, u64 c, u64 Mp, u64 &k0, u64 &nbits) {
   nbits = ke > kb ? ke - kb : 0;
 }
 
+// lane_span for every lane of one window with two divisions in all: for 0 <= c < Mp,
+// ceil((x - c) / Mp) = floor(x / Mp) + (x mod Mp > c ? 1 : 0), and x <= c gives 0.
+struct WindowSpan {
+  u64 ql, rl, qh, rh;
+  WindowSpan(u64 lo, u64 hi, u64 Mp) : ql(lo / Mp), rl(lo % Mp), qh(hi / Mp), rh(hi % Mp) {}
+  void lane(u64 c, u64 &k0, u64 &nbits) const {
+    const u64 kb = ql + (rl > c ? 1 : 0), ke = qh + (rh > c ? 1 : 0);
+    k0 = kb;
+    nbits = ke > kb ? ke - kb : 0;
+  }
+};
+
 // The whole host-side plan of one scanner configuration.
 struct Plan {
   u64 M = 0, mmax = 0, sieve = 0;

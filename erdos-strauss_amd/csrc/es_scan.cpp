@@ -343,16 +343,17 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
     if (id < 0) return -1;
     r.layout_Mp = Mp;
     const Layout &L = *layouts_[Mp];
+    const WindowSpan span(r.lo, r.hi, Mp);
     for (u32 i = 0; i < (u32)L.lanes.size(); ++i) {
       u64 k0, nbits;
-      lane_span(r.lo, r.hi, L.lanes[i], Mp, k0, nbits);
+      span.lane(L.lanes[i], k0, nbits);
       for (u64 b = 0; b < nbits; b += maxb)
         tiles.push_back(TileDesc{k0 + b, (u32)std::min<u64>(maxb, nbits - b), i, (u32)w, (u32)id});
     }
   }
   // Longest tiles first: the persistent queue then ends with the short ones.
-  std::stable_sort(tiles.begin(), tiles.end(),
-                   [](const TileDesc &a, const TileDesc &b) { return a.nbits > b.nbits; });
+  auto longer = [](const TileDesc &a, const TileDesc &b) { return a.nbits > b.nbits; };
+  if (!std::is_sorted(tiles.begin(), tiles.end(), longer)) std::stable_sort(tiles.begin(), tiles.end(), longer);
   tm.tiles = tiles.size();
   const double t1 = now_ms();
   tm.plan_ms = t1 - t0;

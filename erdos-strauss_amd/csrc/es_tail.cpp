@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -214,11 +215,43 @@ u128 least_witness(u64 p, const Fac &fa, u128 q, unsigned r, unsigned target) {
   return best[target] == NONE ? 0 : best[target];
 }
 
+// Decimal digits of v written backwards ending at `end`, two at a time from a table
+// (64-bit division by a constant compiles to a multiply); returns the first digit.
+char *put_u64(char *end, u64 v) {
+  static const char kPairs[] =
+      "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
+      "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
+      "8081828384858687888990919293949596979899";
+  while (v >= 100) {
+    const u64 q = v / 100;
+    const unsigned r = (unsigned)(v - q * 100);
+    end -= 2;
+    memcpy(end, kPairs + 2 * r, 2);
+    v = q;
+  }
+  if (v >= 10) {
+    end -= 2;
+    memcpy(end, kPairs + 2 * v, 2);
+  } else {
+    *--end = (char)('0' + v);
+  }
+  return end;
+}
+
+// As put_u64 for 128-bit v: one 128-bit division splits off the low 19 digits.
+char *put_u128(char *end, u128 v) {
+  if (!(v >> 64)) return put_u64(end, (u64)v);
+  const u64 kTen19 = 10000000000000000000ull;
+  const u64 lo = (u64)(v % kTen19);
+  char *start = put_u64(end, lo);
+  while (start > end - 19) *--start = '0';          // zero-pad the low part to 19 digits
+  return put_u128(start, v / kTen19);
+}
+
 void append_u128(std::string &s, u128 v) {
   char buf[48];
-  int n = 0;
-  do { buf[n++] = (char)('0' + (int)(v % 10)); v /= 10; } while (v);
-  while (n--) s.push_back(buf[n]);
+  char *b = put_u128(buf + sizeof buf, v);
+  s.append(b, buf + sizeof buf - b);
 }
 
 }  // namespace
@@ -328,11 +361,19 @@ bool solve_direct(u64 p, std::string &line) {
 }
 
 void format_rung(u64 p, u64 A, unsigned r, u128 u, std::string &line) {
-  char head[96];
-  snprintf(head, sizeof head, "RUNG %llu %llu %u ", (unsigned long long)p, (unsigned long long)A, r);
-  line = head;
-  append_u128(line, u);
-  line.push_back('\n');
+  char buf[128];
+  char *end = buf + sizeof buf, *b = end;
+  *--b = '\n';
+  b = put_u128(b, u);
+  *--b = ' ';
+  b = put_u64(b, r);
+  *--b = ' ';
+  b = put_u64(b, A);
+  *--b = ' ';
+  b = put_u64(b, p);
+  b -= 5;
+  memcpy(b, "RUNG ", 5);
+  line.assign(b, end - b);
 }
 
 int certify_prime(u64 p, std::string &line) {

@@ -81,3 +81,23 @@ def test_stage_d_fp64_mulmod_is_exact(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", src, "-o", exe], check=True)
     out = subprocess.run([exe, "2000000", "7"], stdout=subprocess.PIPE, check=True).stdout.decode()
     assert out.strip() == "ok 2000000", out
+
+
+def test_certificate_line_formatting(tmp_path):
+    """The table-driven decimal formatter of certificate lines against printf-style
+    formatting with digit-by-digit 128-bit division (edge values, 2 million random lines)."""
+    src = os.path.join(HERE, "replay", "format_check.cpp")
+    exe = str(tmp_path / "format_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, os.path.join(CSRC, "es_tail.cpp"),
+                    os.path.join(CSRC, "es_plan.cpp"), "-o", exe, "-lpthread"], check=True)
+    out = subprocess.run([exe], stdout=subprocess.PIPE, check=True).stdout.decode()
+    assert out.strip() == "format ok", out
+
+
+def test_window_lane_spans(tmp_path):
+    """Per-window lane spans (two divisions per window) equal the per-lane formula."""
+    src = os.path.join(HERE, "replay", "span_check.cpp")
+    exe = str(tmp_path / "span_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True)
+    out = subprocess.run([exe], stdout=subprocess.PIPE, check=True).stdout.decode()
+    assert out.strip() == "span ok", out
