@@ -332,7 +332,15 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
   tm = ScanTiming();
   out.assign(wins.size(), WindowRecord());
   const u32 maxb = tune_.launch.max_bits;
-  std::vector<TileDesc> tiles;
+  // Tiles go out longest first, so the persistent queue ends with the short ones. Only
+  // load balance depends on that order (never a result), so tiles are placed straight
+  // into buckets of nbits / 64 (counted in a first pass) instead of being sorted.
+  const u32 nbk = (maxb >> 6) + 1;
+  auto bucket = [&](u32 nb) { return nbk - 1 - (nb >> 6); };
+  struct LaneSpan { u64 k0, nbits; };
+  std::vector<LaneSpan> spans;
+  std::vector<int> ids(wins.size(), -1);
+  std::vector<u32> start(nbk + 1, 0);
   for (size_t w = 0; w < wins.size(); ++w) {
     WindowRecord &r = out[w];
     r.lo = wins[w].lo;
@@ -341,19 +349,31 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
     const u64 Mp = choose_layout(r.hi - r.lo);
     const int id = ensure_layout(Mp, msg);
     if (id < 0) return -1;
+    ids[w] = id;
     r.layout_Mp = Mp;
     const Layout &L = *layouts_[Mp];
     const WindowSpan span(r.lo, r.hi, Mp);
     for (u32 i = 0; i < (u32)L.lanes.size(); ++i) {
-      u64 k0, nbits;
-      span.lane(L.lanes[i], k0, nbits);
-      for (u64 b = 0; b < nbits; b += maxb)
-        tiles.push_back(TileDesc{k0 + b, (u32)std::min<u64>(maxb, nbits - b), i, (u32)w, (u32)id});
+      LaneSpan ls;
+      span.lane(L.lanes[i], ls.k0, ls.nbits);
+      spans.push_back(ls);
+      for (u64 b = 0; b < ls.nbits; b += maxb) ++start[bucket((u32)std::min<u64>(maxb, ls.nbits - b)) + 1];
     }
   }
-  // Longest tiles first: the persistent queue then ends with the short ones.
-  auto longer = [](const TileDesc &a, const TileDesc &b) { return a.nbits > b.nbits; };
-  if (!std::is_sorted(tiles.begin(), tiles.end(), longer)) std::stable_sort(tiles.begin(), tiles.end(), longer);
+  for (u32 b = 0; b < nbk; ++b) start[b + 1] += start[b];
+  std::vector<TileDesc> tiles(start[nbk]);
+  size_t si = 0;
+  for (size_t w = 0; w < wins.size(); ++w) {
+    if (ids[w] < 0) continue;
+    const u32 nl = (u32)layouts_[out[w].layout_Mp]->lanes.size();
+    for (u32 i = 0; i < nl; ++i, ++si) {
+      const LaneSpan &ls = spans[si];
+      for (u64 b = 0; b < ls.nbits; b += maxb) {
+        const u32 nb = (u32)std::min<u64>(maxb, ls.nbits - b);
+        tiles[start[bucket(nb)]++] = TileDesc{ls.k0 + b, nb, i, (u32)w, (u32)ids[w]};
+      }
+    }
+  }
   tm.tiles = tiles.size();
   const double t1 = now_ms();
   tm.plan_ms = t1 - t0;
