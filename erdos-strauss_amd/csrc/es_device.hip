@@ -303,19 +303,18 @@ __device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 n) {
   return (s < a || s >= n) ? s - n : s;
 }
 
-// x * y mod n for x, y < n < 2^50, exact, in FP64 (full-rate FMAs instead of the
-// quarter-rate 32-bit multiplies of 64-bit Montgomery). h + l == x*y exactly (FMA
-// residual); q is floor(x*y/n) up to +-1, so h - q*n is an integer below 2^52 in
-// magnitude, computed exactly by one FMA; one correction each way finishes.
-__device__ __forceinline__ u64 mulmod_f64(u64 x, u64 y, double nd, double ninv) {
-  const double a = (double)x, b = (double)y;
+// a * b mod n on FP64 values holding integers, n < 2^50, with signed residues (full-rate
+// FMAs instead of the quarter-rate 32-bit multiplies of 64-bit Montgomery): h + l == a*b
+// exactly (FMA residual); q = rint(a*b/n) makes h - q*n exact (one FMA), and
+// r = a*b - q*n has |r| <= (1/2 + 3*2^-53*|a*b|/n) * n. So |a|, |b| < n + 1024 give
+// |r| < 7n/8 (a chain of products stays below n with no correction), and |a| <= 2n,
+// |b| < 2n give |r| < 2n (the product of differences stays below 2n with no correction
+// either).
+__device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, double ninv) {
   const double h = a * b;
   const double l = __fma_rn(a, b, -h);
-  const double q = floor(h * ninv);
-  double r = __fma_rn(-q, nd, h) + l;
-  if (r < 0) r += nd;
-  if (r >= nd) r -= nd;
-  return (u64)r;
+  const double q = rint(h * ninv);
+  return __fma_rn(-q, nd, h) + l;
 }
 
 constexpr u64 kF64Mod = 1ull << 50;
@@ -341,18 +340,20 @@ __device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
   return false;
 }
 
-// The same test in plain residues with mulmod_f64, for n < 2^50.
-__device__ bool strong_probable_f64(u64 n, u64 a, u64 d, u32 s) {
-  const double nd = (double)n, ninv = 1.0 / nd;
-  u64 x = 1, b = a % n;
+// The same test for n < 2^50 in signed FP64 residues: every operand of mulmod_sgn stays
+// in (-n, n), so square-and-multiply needs no correction, and x == 1, x == -1 (mod n)
+// are each one of two representatives in (-n, n).
+__device__ bool strong_probable_sgn(u64 n, u32 a, u64 d, u32 s) {
+  const double nd = (double)n, ninv = 1.0 / nd, m1 = nd - 1.0, p1 = 1.0 - nd;
+  double x = 1.0, b = (double)a;
   for (u64 e = d; e; e >>= 1) {
-    if (e & 1) x = mulmod_f64(x, b, nd, ninv);
-    b = mulmod_f64(b, b, nd, ninv);
+    if (e & 1) x = mulmod_sgn(x, b, nd, ninv);
+    b = mulmod_sgn(b, b, nd, ninv);
   }
-  if (x == 1 || x == n - 1) return true;
+  if (x == 1.0 || x == p1 || x == -1.0 || x == m1) return true;
   for (u32 i = 1; i < s; ++i) {
-    x = mulmod_f64(x, x, nd, ninv);
-    if (x == n - 1) return true;
+    x = mulmod_sgn(x, x, nd, ninv);
+    if (x == -1.0 || x == m1) return true;
   }
   return false;
 }
@@ -397,7 +398,7 @@ __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__
         u64 d = n - 1;
         const u32 s = (u32)__builtin_ctzll(d);
         d >>= s;
-        v = strong_probable_f64(n, 2, d, s) ? kPending : kComposite;
+        v = strong_probable_sgn(n, 2, d, s) ? kPending : kComposite;
       } else {
         u64 ninv = n;                         // n^{-1} mod 2^64 (Newton), then negate
         for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
@@ -430,7 +431,7 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
     d >>= s;
     bool ok;
     if (n < kF64Mod) {
-      ok = strong_probable_f64(n, c_small_p[b], d, s);
+      ok = strong_probable_sgn(n, (u32)c_small_p[b], d, s);
     } else {
       u64 ninv = n;
       for (int k = 0; k < 6; ++k) ninv *= 2 - n * ninv;
@@ -467,9 +468,10 @@ constexpr u32 kRungFound = 1, kRungNone = 2, kRungHost = 3;
 constexpr u32 kRungWaves = 4;     // waves per prime = rungs in flight
 constexpr u32 kRungSlots = 64;    // candidate rungs r0 + 4j <= 255
 constexpr u32 kSmallFac = 16;     // odd primes < 2^16 dividing one A < 2^62: at most 15
+constexpr u32 kSievePer = 35;     // table loads in flight per sieving thread (6541 primes / 192 threads)
 
 struct TrialTab {
-  const uint4 *qm;  // odd primes 3 .. 65521 ascending: {q, floor(2^32/q), 2^32 mod q, 0}
+  const uint2 *qm;  // odd primes 3 .. 65521 ascending: {q | (2^32 mod q) << 16, floor(2^32/q)}
   const u64 *inv;   // q^{-1} mod 2^64
   const u64 *lim;   // floor((2^64-1)/q)
   u32 n;
@@ -539,7 +541,7 @@ __device__ bool wave_is_prime(u64 n, u32 lane) {
   bool ok = true;
   if (lane < 12) {
     if (n < kF64Mod) {
-      ok = strong_probable_f64(n, c_small_p[lane], d, s);
+      ok = strong_probable_sgn(n, (u32)c_small_p[lane], d, s);
     } else {
       const u64 ninv = neg_inv(n), one = (0 - n) % n;
       ok = strong_probable(n, ninv, one, c_small_p[lane], d, s);
@@ -634,18 +636,6 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
   return kRhoFail;
 }
 
-
-// a * b mod n on FP64 values holding integers, n < 2^50, with signed residues:
-// q = rint(a*b/n) makes r = a*b - q*n exact (FMA residual, one exact FMA) with
-// |r| <= (1/2 + 3*2^-53*|a*b|/n) * n. So |a|, |b| < n + 1024 give |r| < 7n/8 (the walk
-// stays below n with no correction), and |a| <= 2n, |b| < 2n give |r| < 2n (the product
-// of differences stays below 2n with no correction either).
-__device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, double ninv) {
-  const double h = a * b;
-  const double l = __fma_rn(a, b, -h);
-  const double q = rint(h * ninv);
-  return __fma_rn(-q, nd, h) + l;
-}
 
 constexpr int kRhoWalks = 2;   // independent walks per lane (interleaved for ILP)
 
@@ -755,7 +745,7 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
   u64 pw = 1;
   if (lane < nsmall) {
     const u32 i = small[lane];
-    const u32 q = T.qm[i].x;
+    const u32 q = T.qm[i].x & 0xFFFFu;
     const u64 qi = T.inv[i], ql = T.lim[i];
     u64 m = n;
     u32 e = 0;
@@ -847,8 +837,8 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
 
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
-                                                          const u8 *__restrict__ flags, TrialTab T,
-                                                          RungOut *__restrict__ out) {
+                                                          u8 *__restrict__ flags, const u64 *__restrict__ one_in,
+                                                          TrialTab T, RungOut *__restrict__ out) {
   __shared__ u128 best[kRungWaves][256], nxt[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -856,11 +846,13 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   __shared__ u32 next_k, found_k, host_k, stop_k;
   __shared__ RungOut wres[kRungWaves];
   __shared__ RhoShare rs;
+  __shared__ u32 mr_composite;
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
   for (u32 idx = blockIdx.x; idx < n_; idx += gridDim.x) {
-    if (!flags[idx]) continue;
+    const u8 f0 = flags[idx];
+    if (f0 == kComposite) continue;
     const u64 p = vals[idx];
     u32 r0 = (u32)((4 - (p & 3)) & 3);
     if (r0 == 0) r0 = 4;
@@ -872,30 +864,56 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       next_k = 0;
       found_k = host_k = stop_k = kRungSlots;
       rs = RhoShare{0, 0, 0, 0, 0, 0};
+      mr_composite = 0;
     }
     __syncthreads();
-    // Sieve the 64 consecutive A_j by every odd prime below 2^16 (four table loads in
-    // flight per thread before any is used: the loop is otherwise one load latency per
-    // prime).
-    for (u32 i0 = tid; i0 < T.n; i0 += 4 * blockDim.x) {
-      uint4 c[4];
-#pragma unroll
-      for (u32 u = 0; u < 4; ++u) {
-        const u32 i = i0 + u * blockDim.x;
-        c[u] = i < T.n ? T.qm[i] : make_uint4(1u, 0u, 0u, 0u);
+    // A value the base-2 pass left pending gets its eleven odd prime bases here, one per
+    // lane of wave 0, while the other waves sieve.
+    const bool pending = f0 == kPending;
+    if (pending && wv == 0) {
+      bool ok = true;
+      if (lane < 11) {
+        u64 d = p - 1;
+        const u32 s2 = (u32)__builtin_ctzll(d);
+        d >>= s2;
+        const u32 a = (u32)c_small_p[lane + 1];
+        ok = p < kF64Mod ? strong_probable_sgn(p, a, d, s2)
+                         : strong_probable(p, neg_inv(p), one_in[idx], a, d, s2);
       }
+      if (__ballot(!ok) && lane == 0) mr_composite = 1;
+    }
+    // Sieve the 64 consecutive A_j by every odd prime below 2^16. Each sieving thread
+    // issues all its table loads before using any (the loop is otherwise one load latency
+    // per prime).
+    const u32 w0 = pending ? 64u : 0u;
+    if (tid >= w0) {
+      const u32 me = tid - w0, nw = blockDim.x - w0;
+      for (u32 i0 = me; i0 < T.n; i0 += kSievePer * nw) {
+        uint2 c[kSievePer];
 #pragma unroll
-      for (u32 u = 0; u < 4; ++u) {
-        const u32 i = i0 + u * blockDim.x;
-        if (i >= T.n) continue;
-        const u32 x = mod64(A0, c[u].x, c[u].y, c[u].z);
-        for (u32 j = x ? c[u].x - x : 0; j < nj; j += c[u].x) {
-          const u32 s = atomicAdd(&nfac[j], 1u);
-          if (s < kSmallFac) fac[j][s] = (u16)i;
+        for (u32 u = 0; u < kSievePer; ++u) {
+          const u32 i = i0 + u * nw;
+          c[u] = T.qm[min(i, T.n - 1u)];   // unconditional, so every load is in flight at once
+        }
+#pragma unroll
+        for (u32 u = 0; u < kSievePer; ++u) {
+          const u32 i = i0 + u * nw;
+          if (i >= T.n) continue;
+          const u32 q = c[u].x & 0xFFFFu;
+          const u32 x = mod64(A0, q, c[u].y, c[u].x >> 16);
+          for (u32 j = x ? q - x : 0; j < nj; j += q) {
+            const u32 s = atomicAdd(&nfac[j], 1u);
+            if (s < kSmallFac) fac[j][s] = (u16)i;
+          }
         }
       }
     }
     __syncthreads();
+    if (mr_composite) {                    // uniform: read after the barrier
+      if (tid == 0) flags[idx] = kComposite;
+      __syncthreads();
+      continue;
+    }
     for (;;) {
       u32 k = 0;
       if (lane == 0) k = atomicAdd(&next_k, 1u);
@@ -1117,26 +1135,27 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
   HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32), hipHostMallocMapped));
   HIP_TRY(hipHostGetDevicePointer((void **)&m.dh_count, m.h_count, 0));
-  // Sieve table for stage D: odd primes q below 2^16 with {q, floor(2^32/q), 2^32 mod q}
-  // (residues of A by q) and q^{-1} mod 2^64, floor((2^64-1)/q) (exact division by q).
+  // Sieve table for stage D: odd primes q below 2^16 packed as {q | (2^32 mod q) << 16,
+  // floor(2^32/q)} (residues of A by q), and q^{-1} mod 2^64, floor((2^64-1)/q) (exact
+  // division by q).
   std::vector<uint8_t> comp(65537, 0);
-  std::vector<uint4> tq;
+  std::vector<uint2> tq;
   std::vector<u64> tinv, tlim;
   for (u32 i = 3; i <= 65536; i += 2) {
     if (comp[i]) continue;
-    tq.push_back(uint4{i, (u32)(0x100000000ull / i), (u32)(0x100000000ull % i), 0});
+    tq.push_back(uint2{i | (u32)(0x100000000ull % i) << 16, (u32)(0x100000000ull / i)});
     u64 x = i;
     for (int k = 0; k < 6; ++k) x *= 2 - (u64)i * x;
     tinv.push_back(x);
     tlim.push_back(~0ull / i);
     for (u32 j = i * i; j <= 65536; j += 2 * i) comp[j] = 1;
   }
-  uint4 *dq = nullptr;
+  uint2 *dq = nullptr;
   u64 *dinv = nullptr, *dlim = nullptr;
-  HIP_TRY(hipMalloc(&dq, tq.size() * sizeof(uint4)));
+  HIP_TRY(hipMalloc(&dq, tq.size() * sizeof(uint2)));
   HIP_TRY(hipMalloc(&dinv, tinv.size() * 8));
   HIP_TRY(hipMalloc(&dlim, tlim.size() * 8));
-  HIP_TRY(hipMemcpy(dq, tq.data(), tq.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dq, tq.data(), tq.size() * sizeof(uint2), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dinv, tinv.data(), tinv.size() * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dlim, tlim.data(), tlim.size() * 8, hipMemcpyHostToDevice));
   m.trial = dev::TrialTab{dq, dinv, dlim, (u32)tq.size()};
@@ -1327,14 +1346,15 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
                                                                   m.d_flags, m.d_one);
     HIP_TRY(hipGetLastError());
-    dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
-                                                                  m.d_flags, m.d_one);
-    HIP_TRY(hipGetLastError());
     if (cfg.device_tail) {
-      dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
-                                                               m.d_flags, m.trial, m.d_rung);
-      HIP_TRY(hipGetLastError());
+      // k_rung finishes the Miller-Rabin bases of the pending values itself.
+      dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(
+          m.d_surv_p, m.d_scalars, (u32)m.surv_cap, m.d_flags, m.d_one, m.trial, m.d_rung);
+    } else {
+      dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
+                                                                    m.d_flags, m.d_one);
     }
+    HIP_TRY(hipGetLastError());
     dev::k_pack<<<std::max(1, cus_), 256, 0, m.stream>>>(m.d_hdr, m.d_tile_counts, m.d_scalars, (u32)m.surv_cap,
                                                          m.d_surv_p, m.d_surv_w, m.d_flags, m.d_rung,
                                                          cfg.device_tail ? 1u : 0u, m.dh_count, m.dh_tile_counts,
@@ -1420,7 +1440,7 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_f, ones.data(), n, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
-  dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32, d_f, m.trial, d_o);
+  dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32, d_f, nullptr, m.trial, d_o);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
@@ -1438,7 +1458,7 @@ int DeviceCtx::mem_info(size_t &free_b, size_t &total_b) {
 size_t DeviceCtx::resident_bytes() const {
   const Impl &m = *impl_;
   const size_t per_surv = sizeof(u64) * 2 + sizeof(u32) + 1 + sizeof(dev::RungOut);
-  return layout_bytes_ + m.trial.n * (sizeof(uint4) + 16) + m.tiles_cap * sizeof(TileDesc) +
+  return layout_bytes_ + m.trial.n * (sizeof(uint2) + 16) + m.tiles_cap * sizeof(TileDesc) +
          m.tiles_cap * sizeof(uint4) + m.surv_cap * per_surv;
 }
 

@@ -1,12 +1,14 @@
-// mulmod_replay.cpp -- test-only CPU replay of the stage-D FP64 modular multiplies
-// against 128-bit integer arithmetic:
-//   mulmod_f64   x * y mod n for x, y < n < 2^50: product and FMA residual, floor
-//                quotient estimate, one exact FMA for h - q*n, one correction each way;
-//   mulmod_sgn   the signed-residue form rho walks in: a * b with q = rint(a*b/n) gives
+// mulmod_replay.cpp -- test-only CPU replay of the device's FP64 modular arithmetic
+// (moduli below 2^50) against 128-bit integer arithmetic:
+//   mulmod_sgn   the signed-residue multiply Miller-Rabin and rho use: a * b with
+//                q = rint(a*b/n) gives
 //                r == a*b (mod n) with |r| <= (1/2 + 3*2^-53*|a*b|/n) * n, exactly, for
 //                |a*b| < 2^102 -- so |r| < n for |a|, |b| < n + 64 (the walk), |r| < 1.25 n
 //                for |a| < n, |b| < 2n + 128, and |r| < 2n for |a| <= 2n, |b| < 2n (the
-//                accumulated product of differences, which therefore needs no correction).
+//                accumulated product of differences, which therefore needs no correction);
+//   strong_probable_sgn  the strong probable-prime test run in those residues with no
+//                correction anywhere in the chain, whose verdict must equal the test in
+//                exact integers for every base 2 .. 37.
 // IEEE-754 binary64 with a fused multiply-add is what the device executes, so std::fma
 // and std::rint (round to nearest even) give the same results bit for bit.
 //
@@ -20,17 +22,6 @@
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
 
-static u64 mulmod_f64(u64 x, u64 y, double nd, double ninv) {
-  const double a = (double)x, b = (double)y;
-  const double h = a * b;
-  const double l = std::fma(a, b, -h);
-  const double q = std::floor(h * ninv);
-  double r = std::fma(-q, nd, h) + l;
-  if (r < 0) r += nd;
-  if (r >= nd) r -= nd;
-  return (u64)r;
-}
-
 static double mulmod_sgn(double a, double b, double nd, double ninv) {
   const double h = a * b;
   const double l = std::fma(a, b, -h);
@@ -39,6 +30,35 @@ static double mulmod_sgn(double a, double b, double nd, double ninv) {
 }
 
 typedef __int128 i128;
+
+static bool strong_probable_sgn(u64 n, unsigned a, u64 d, unsigned s) {
+  const double nd = (double)n, ninv = 1.0 / nd, m1 = nd - 1.0, p1 = 1.0 - nd;
+  double x = 1.0, b = (double)a;
+  for (u64 e = d; e; e >>= 1) {
+    if (e & 1) x = mulmod_sgn(x, b, nd, ninv);
+    b = mulmod_sgn(b, b, nd, ninv);
+  }
+  if (x == 1.0 || x == p1 || x == -1.0 || x == m1) return true;
+  for (unsigned i = 1; i < s; ++i) {
+    x = mulmod_sgn(x, x, nd, ninv);
+    if (x == -1.0 || x == m1) return true;
+  }
+  return false;
+}
+
+static bool strong_probable_int(u64 n, unsigned a, u64 d, unsigned s) {
+  u64 x = 1, b = a % n;
+  for (u64 e = d; e; e >>= 1) {
+    if (e & 1) x = (u64)((u128)x * b % n);
+    b = (u64)((u128)b * b % n);
+  }
+  if (x == 1 || x == n - 1) return true;
+  for (unsigned i = 1; i < s; ++i) {
+    x = (u64)((u128)x * x % n);
+    if (x == n - 1) return true;
+  }
+  return false;
+}
 
 // r must be an integer-valued double with r == a*b (mod n) and |r| < bound * n.
 static int check_sgn(long long a, long long b, u64 n, double bound) {
@@ -64,14 +84,16 @@ static u64 next() {   // splitmix64
   return z ^ (z >> 31);
 }
 
-static int check(u64 x, u64 y, u64 n) {
-  const double nd = (double)n, ninv = 1.0 / nd;
-  const u64 got = mulmod_f64(x, y, nd, ninv), want = (u64)((u128)x * y % n);
-  if (got != want) {
-    printf("mismatch x=%llu y=%llu n=%llu got=%llu want=%llu\n", (unsigned long long)x, (unsigned long long)y,
-           (unsigned long long)n, (unsigned long long)got, (unsigned long long)want);
-    return 1;
-  }
+// The signed-residue test and the exact one agree on n for every base 2 .. 37.
+static int check_mr(u64 n) {
+  u64 d = n - 1;
+  unsigned s = 0;
+  while (!(d & 1)) { d >>= 1; ++s; }
+  for (unsigned a : {2u, 3u, 5u, 7u, 11u, 13u, 17u, 19u, 23u, 29u, 31u, 37u})
+    if (strong_probable_sgn(n, a, d, s) != strong_probable_int(n, a, d, s)) {
+      printf("MR mismatch n=%llu a=%u\n", (unsigned long long)n, a);
+      return 1;
+    }
   return 0;
 }
 
@@ -79,18 +101,19 @@ int main(int argc, char **argv) {
   const u64 trials = argc > 1 ? strtoull(argv[1], 0, 10) : 1000000;
   state = argc > 2 ? strtoull(argv[2], 0, 10) : 1;
   const u64 top = 1ull << 50;
-  // Edges: the largest operands and moduli next to 2^32 and 2^50.
   const u64 ns[] = {(1ull << 32) + 15, top - 1, top - 27, top - 2, (1ull << 49) + 1, 4294967311ull * 65537ull};
-  for (u64 n : ns)
-    for (u64 x : {(u64)0, (u64)1, n - 1, n - 2, n / 2, n / 2 + 1})
-      for (u64 y : {(u64)0, (u64)1, n - 1, n - 2, n / 2, n / 2 + 1})
-        if (check(x, y, n)) return 1;
-  for (u64 t = 0; t < trials; ++t) {
-    const int bits = 33 + (int)(next() % 18);           // n in [2^32, 2^50)
+  // Miller-Rabin chains: primes, strong pseudoprimes to several bases, products of two
+  // primes above 2^16, and random odd moduli (most composite, some prime).
+  const u64 mr_edges[] = {(1ull << 50) - 27, (1ull << 49) - 81, 4294967311ull, 3215031751ull,
+                          2152302898747ull, 3474749660383ull, 341550071728321ull, 65537ull * 65539ull,
+                          4294967291ull * 262139ull, 1125899906842597ull, 999999999999989ull};
+  for (u64 n : mr_edges)
+    if (check_mr(n)) return 1;
+  for (u64 t = 0; t < trials / 8; ++t) {
+    const int bits = 33 + (int)(next() % 18);
     u64 n = (next() >> (64 - bits)) | (1ull << (bits - 1)) | 1;
     if (n >= top) n = top - 1;
-    const u64 x = next() % n, y = (t & 7) == 0 ? n - 1 - (next() & 255) % n : next() % n;
-    if (check(x, y, n)) return 1;
+    if (check_mr(n)) return 1;
   }
   // Signed residues: the walk (|a|, |b| < n + 64) and the difference products.
   for (u64 n : ns) {

@@ -74,8 +74,10 @@ def test_replay_matches_oracle_every_layout(replay_bin, name, lo, hi, M, mmax, s
 
 
 def test_stage_d_fp64_mulmod_is_exact(tmp_path):
-    """The device rho's FP64 modular multiply (moduli below 2^50) equals 128-bit integer
-    arithmetic on edge operands and 2 million random ones."""
+    """The device's signed FP64 modular multiply (moduli below 2^50) stays within its
+    stated bounds and congruent under 128-bit integer arithmetic (edge operands, 2 million
+    random ones), and the Miller-Rabin chain run in it with no corrections gives the exact
+    test's verdict for every base 2 .. 37 (edge moduli, strong pseudoprimes, 250000 random)."""
     src = os.path.join(HERE, "replay", "mulmod_replay.cpp")
     exe = str(tmp_path / "mulmod_replay")
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", src, "-o", exe], check=True)
