@@ -465,10 +465,10 @@ struct RungOut {
   u32 r, status;
 };
 constexpr u32 kRungFound = 1, kRungNone = 2, kRungHost = 3;
-constexpr u32 kRungWaves = 4;     // waves per prime = rungs in flight
+constexpr u32 kRungWaves = 8;     // waves per prime = rungs in flight (two per SIMD)
 constexpr u32 kRungSlots = 64;    // candidate rungs r0 + 4j <= 255
 constexpr u32 kSmallFac = 16;     // odd primes < 2^16 dividing one A < 2^62: at most 15
-constexpr u32 kSievePer = 35;     // table loads in flight per sieving thread (6541 primes / 192 threads)
+constexpr u32 kSievePer = 15;     // table loads in flight per sieving thread (6541 primes / 448 threads)
 
 struct TrialTab {
   const uint2 *qm;  // odd primes 3 .. 65521 ascending: {q | (2^32 mod q) << 16, floor(2^32/q)}
@@ -794,11 +794,16 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
 __device__ __forceinline__ u32 small_mod(u64 x, u32 r, u32 mg, u32 c32) { return mod64(x, r, mg, c32); }
 
 // Least witness of rung r for q = pA (A factored in F): u | (pA)^2, u == target (mod r),
-// u <= q, or ~0 if there is none. best/nxt are this wave's residue tables (r entries).
+// u <= q, or ~0 if there is none. best[t] is the least divisor so far (of the prime powers
+// absorbed) that is == t (mod r) and <= q; absorbing pr^E makes it min over j <= E of
+// best[t / pr^j] * pr^j. Lane l owns residues l, l + 64, l + 128, l + 192: four
+// independent chains per lane, with the powers pr^j shared by the wave. A wave runs in
+// lockstep, so all its reads of best are done before its writes and best is updated in
+// place.
 __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u32 target, u128 q,
-                              u128 *best, u128 *nxt, u32 lane) {
+                              u128 *best, u32 lane) {
   const u128 INF = ~(u128)0;
-  for (u32 t = lane; t < r; t += 64) best[t] = t == 1 % r ? (u128)1 : INF;
+  for (u32 t = lane; t < 256; t += 64) best[t] = t == 1 % r ? (u128)1 : INF;
   wave_sync();
   // Absorb p^2, then every prime power of A^2.
   for (u32 fi = 0; fi <= F.k; ++fi) {
@@ -811,23 +816,35 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
     for (u32 x = lane; x < r; x += 64) if (mod32(pm * x, r, mg) == one) pinv = x + 1;
     const u64 bal = __ballot(pinv != 0);
     pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
-    for (u32 t = lane; t < r; t += 64) {
-      u128 acc = INF, pw = 1;
-      u32 s = t;
-      for (u32 j = 0;; ++j) {
-        const u128 b = best[s];
+    u32 s[4];
+    u128 acc[4];
+#pragma unroll
+    for (u32 w = 0; w < 4; ++w) {
+      const u32 t = lane + 64 * w;
+      s[w] = t < r ? t : 0u;
+      acc[w] = INF;
+    }
+    u128 pw = 1;
+    for (u32 j = 0;; ++j) {
+#pragma unroll
+      for (u32 w = 0; w < 4; ++w) {
+        if (64 * w >= r) break;               // uniform: chains beyond r do not exist
+        const u128 b = best[s[w]];
         u128 v;
-        if (b != INF && mul_le(b, pw, q, v) && v < acc) acc = v;
-        if (j == E) break;
-        u128 npw;
-        if (!mul_le(pw, (u128)pr, q, npw)) break;
-        pw = npw;
-        s = mod32(s * pinv, r, mg);   // < 255 * 255, and mg = floor(2^32 / r)
+        if (b != INF && mul_le(b, pw, q, v) && v < acc[w]) acc[w] = v;
       }
-      nxt[t] = acc;
+      if (j == E) break;
+      u128 npw;
+      if (!mul_le(pw, (u128)pr, q, npw)) break;   // pr^(j+1) > q: no further term fits
+      pw = npw;
+#pragma unroll
+      for (u32 w = 0; w < 4; ++w)
+        if (64 * w < r) s[w] = mod32(s[w] * pinv, r, mg);   // < 255 * 255
     }
     wave_sync();
-    for (u32 t = lane; t < r; t += 64) best[t] = nxt[t];
+#pragma unroll
+    for (u32 w = 0; w < 4; ++w)
+      if (lane + 64 * w < r) best[lane + 64 * w] = acc[w];
     wave_sync();
   }
   const u128 u = best[target];
@@ -835,11 +852,27 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
   return u;
 }
 
+// Helps the block's cofactor split in progress, if there is one this wave has not helped
+// yet, until that split is over; false if there was none.
+__device__ bool help_split(RhoShare *rs, u32 &seen, u32 wv, u32 lane) {
+  volatile RhoShare *v = rs;
+  const u32 gen = __builtin_amdgcn_readfirstlane(v->gen);
+  if (!__builtin_amdgcn_readfirstlane(v->busy) || gen == seen) return false;
+  seen = gen;
+  const u64 n = __builtin_amdgcn_readfirstlane((u32)v->n) |
+                ((u64)__builtin_amdgcn_readfirstlane((u32)(v->n >> 32)) << 32);
+  if (__builtin_amdgcn_readfirstlane(v->gen) == gen && n > 1) {
+    const RhoRole ro{rs, 1u, gen, 128u * (wv + 1)};
+    (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
                                                           TrialTab T, RungOut *__restrict__ out) {
-  __shared__ u128 best[kRungWaves][256], nxt[kRungWaves][256];
+  __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
   __shared__ u32 nfac[kRungSlots];
@@ -895,16 +928,32 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
           const u32 i = i0 + u * nw;
           c[u] = T.qm[min(i, T.n - 1u)];   // unconditional, so every load is in flight at once
         }
+        // First A_j each prime divides (64 = none in this slot) ...
+        u32 j0[kSievePer];
 #pragma unroll
         for (u32 u = 0; u < kSievePer; ++u) {
-          const u32 i = i0 + u * nw;
-          if (i >= T.n) continue;
           const u32 q = c[u].x & 0xFFFFu;
           const u32 x = mod64(A0, q, c[u].y, c[u].x >> 16);
-          for (u32 j = x ? q - x : 0; j < nj; j += q) {
-            const u32 s = atomicAdd(&nfac[j], 1u);
-            if (s < kSmallFac) fac[j][s] = (u16)i;
-          }
+          j0[u] = i0 + u * nw < T.n ? (x ? q - x : 0u) : 64u;
+        }
+        // ... then every slot claim issued before any is used (a prime above 64 hits at
+        // most one A_j, yet some lane of a wave almost always hits one), and the further
+        // hits of the primes below 64 last.
+        u32 sl[kSievePer];
+#pragma unroll
+        for (u32 u = 0; u < kSievePer; ++u)
+          if (j0[u] < nj) sl[u] = atomicAdd(&nfac[j0[u]], 1u);
+#pragma unroll
+        for (u32 u = 0; u < kSievePer; ++u)
+          if (j0[u] < nj && sl[u] < kSmallFac) fac[j0[u]][sl[u]] = (u16)(i0 + u * nw);
+#pragma unroll
+        for (u32 u = 0; u < kSievePer; ++u) {
+          const u32 q = c[u].x & 0xFFFFu;
+          if (j0[u] < nj && q < nj)
+            for (u32 j = j0[u] + q; j < nj; j += q) {
+              const u32 s = atomicAdd(&nfac[j], 1u);
+              if (s < kSmallFac) fac[j][s] = (u16)(i0 + u * nw);
+            }
         }
       }
     }
@@ -914,7 +963,11 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       __syncthreads();
       continue;
     }
+    // Rungs in order. Between rungs a wave first helps a split in progress: that split
+    // belongs to a smaller rung than any it could take, so it is on the critical path.
+    u32 seen = 0;
     for (;;) {
+      help_split(&rs, seen, wv, lane);
       u32 k = 0;
       if (lane == 0) k = atomicAdd(&next_k, 1u);
       k = __shfl(k, 0, 64);
@@ -933,7 +986,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
         if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
         break;
       }
-      const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], nxt[wv], lane);
+      const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], lane);
       if (u != ~(u128)0) {
         if (lane == 0) {
           wres[wv] = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
@@ -946,25 +999,8 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     // A wave done with its rungs helps whichever wave still splits a cofactor; all leave
     // once every wave is done with its rungs (each rung loop ends: its rho is bounded).
     if (lane == 0) atomicAdd(&rs.done, 1u);
-    {
-      volatile RhoShare *v = &rs;
-      u32 seen = 0;
-      for (;;) {
-        if (__builtin_amdgcn_readfirstlane(v->done) >= kRungWaves) break;
-        const u32 gen = __builtin_amdgcn_readfirstlane(v->gen);
-        if (__builtin_amdgcn_readfirstlane(v->busy) && gen != seen) {
-          seen = gen;
-          const u64 n = __builtin_amdgcn_readfirstlane((u32)v->n) |
-                        ((u64)__builtin_amdgcn_readfirstlane((u32)(v->n >> 32)) << 32);
-          if (__builtin_amdgcn_readfirstlane(v->gen) == gen && n > 1) {
-            const RhoRole ro{&rs, 1u, gen, 128u * (wv + 1)};
-            (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
-          }
-        } else {
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-    }
+    while (__builtin_amdgcn_readfirstlane(((volatile RhoShare *)&rs)->done) < kRungWaves)
+      if (!help_split(&rs, seen, wv, lane)) __builtin_amdgcn_s_sleep(2);
     __syncthreads();
     if (tid == 0) {
       RungOut res{0, 0, 0, 0, kRungNone};
