@@ -281,8 +281,8 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
 // Zeroes the survivor count and the tile queue (a fill kernel rather than a memset, so
 // a captured graph holds no memset node). Every other per-batch output is written in
 // full by the kernels, so one captured graph serves every batch that fits the buffers.
-__global__ void k_reset(u32 *surv_count, u32 *queue) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) { *surv_count = 0; *queue = 0; }
+__global__ void k_reset(u32 *scalars) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) scalars[threadIdx.x] = 0;   // counts and queue heads
 }
 
 // ------------------------------------------------------------------ Miller-Rabin
@@ -522,10 +522,10 @@ __device__ __forceinline__ u64 inv64(u64 n) {
 
 __device__ __forceinline__ u64 neg_inv(u64 n) { return 0 - inv64(n); }
 
-// a * 2^64 mod n for a < 2^16, by double-and-add from one = 2^64 mod n.
+// a * 2^64 mod n for a < 2^32, by double-and-add from one = 2^64 mod n.
 __device__ __forceinline__ u64 small_to_mont(u32 a, u64 one, u64 n) {
   u64 am = 0;
-  for (int bit = 15; bit >= 0; --bit) {
+  for (int bit = 31; bit >= 0; --bit) {
     am = add_mod(am, am, n);
     if ((a >> bit) & 1) am = add_mod(am, one, n);
   }
@@ -606,7 +606,15 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
   bool alive = true;
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
     x = y;
-    for (u32 i = 0; i < r; ++i) y = f(y);
+    // Advance r steps, looking for a stop or a posted factor every 64.
+    for (u32 i = 0; i < r; ++i) {
+      y = f(y);
+      if ((i & 63) == 63) {
+        if (stop && *stop < k) return kRhoStop;
+        const u64 h = rho_coop(ro, n);
+        if (h) return h;
+      }
+    }
     for (u32 kk = 0; kk < r; kk += 64) {
       ys = y;
       const u32 lim = min(64u, r - kk);
@@ -637,7 +645,7 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
 }
 
 
-constexpr int kRhoWalks = 2;   // independent walks per lane (interleaved for ILP)
+constexpr int kRhoWalks = 8;   // independent walks per lane: FP64 latency is ~8 issue slots
 
 // wave_rho for n < 2^50 (same contract): 64 x kRhoWalks walks x -> x^2 + c with
 // c = cbase + lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts
@@ -657,9 +665,15 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
 #pragma unroll
     for (int w = 0; w < kRhoWalks; ++w) x[w] = y[w];
+    // Advance r steps, looking for a stop or a posted factor every 16.
     for (u32 i = 0; i < r; ++i) {
 #pragma unroll
       for (int w = 0; w < kRhoWalks; ++w) y[w] = f(y[w], c[w]);
+      if ((i & 15) == 15) {
+        if (stop && *stop < k) return kRhoStop;
+        const u64 h = rho_coop(ro, n);
+        if (h) return h;
+      }
     }
     for (u32 kk = 0; kk < r; kk += 64) {
       const u32 lim = min(64u, r - kk);
@@ -669,27 +683,47 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
 #pragma unroll
         for (int w = 0; w < kRhoWalks; ++w) {
           y[w] = f(y[w], c[w]);
-          acc[w] = mulmod_sgn(acc[w], x[w] - y[w], nd, ninv);   // |acc| < 2n
+          acc[w] = mulmod_sgn(acc[w], alive[w] ? x[w] - y[w] : 1.0, nd, ninv);   // |acc| < 2n
+        }
+        if ((i & 15) == 15) {              // a stop or a factor another wave posted
+          if (stop && *stop < k) return kRhoStop;
+          const u64 h = rho_coop(ro, n);
+          if (h) return h;
         }
       }
-      u64 g = 1;
+      // One gcd per batch, of the product of every walk's accumulator (the chain of
+      // products stays below 2n in magnitude too).
+      double all = acc[0];
+#pragma unroll
+      for (int w = 1; w < kRhoWalks; ++w) all = mulmod_sgn(all, acc[w], nd, ninv);
+      u64 g = gcd_u64((u64)fabs(all), n);
+      if (g == n) {
+        // Some walk overshot (its product is 0 mod n): per-walk gcds, and a step-by-step
+        // replay of the batch for each walk that overshot; a walk whose replay finds no
+        // proper factor is dead from then on (its accumulator stays 1).
+        g = 1;
+#pragma unroll
+        for (int w = 0; w < kRhoWalks; ++w) {
+          u64 gw = alive[w] ? gcd_u64((u64)fabs(acc[w]), n) : 1;
+          if (gw == n) {
+            gw = 1;
+            for (u32 i = 0; i < lim && gw == 1; ++i) {
+              ys[w] = f(ys[w], c[w]);
+              const u64 dd = (u64)fabs(x[w] - ys[w]);
+              gw = gcd_u64(dd ? dd : n, n);
+            }
+            if (gw == n) gw = 1;
+            if (gw == 1) {
+              alive[w] = false;
+              acc[w] = 1;
+            }
+          }
+          if (g == 1) g = gw;
+        }
+      }
       bool any = false;
 #pragma unroll
-      for (int w = 0; w < kRhoWalks; ++w) {
-        u64 gw = alive[w] ? gcd_u64((u64)fabs(acc[w]), n) : 1;
-        if (gw == n) {                    // the batch overshot: replay it step by step
-          gw = 1;
-          for (u32 i = 0; i < lim && gw == 1; ++i) {
-            ys[w] = f(ys[w], c[w]);
-            const u64 dd = (u64)fabs(x[w] - ys[w]);
-            gw = gcd_u64(dd ? dd : n, n);
-          }
-          if (gw == n) gw = 1;
-          if (gw == 1) alive[w] = false;
-        }
-        if (g == 1) g = gw;
-        any |= alive[w];
-      }
+      for (int w = 0; w < kRhoWalks; ++w) any |= alive[w];
       const u64 found = __ballot(g != 1);
       if (found) return rho_found(ro, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
       if (__ballot(any) == 0) return kRhoFail;
@@ -793,6 +827,16 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
 
 __device__ __forceinline__ u32 small_mod(u64 x, u32 r, u32 mg, u32 c32) { return mod64(x, r, mg, c32); }
 
+// pm^{-1} mod r for a unit pm (every prime power's residue on a rung that is not skipped
+// is one): lane x tests x, x + 64, ...; the wave takes the lane that found it.
+__device__ __forceinline__ u32 unit_inverse(u32 pm, u32 r, u32 mg, u32 lane) {
+  u32 inv = 0;
+  const u32 one = r == 1 ? 0u : 1u;
+  for (u32 x = lane; x < r; x += 64) if (mod32(pm * x, r, mg) == one) inv = x + 1;
+  const u64 bal = __ballot(inv != 0);
+  return bal ? __shfl(inv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
+}
+
 // Least witness of rung r for q = pA (A factored in F): u | (pA)^2, u == target (mod r),
 // u <= q, or ~0 if there is none. best[t] is the least divisor so far (of the prime powers
 // absorbed) that is == t (mod r) and <= q; absorbing pr^E makes it min over j <= E of
@@ -810,12 +854,7 @@ __device__ u128 least_witness(u64 p, const WaveFac &F, u32 r, u32 mg, u32 c32, u
     const u64 pr = fi == 0 ? p : F.p[fi - 1];
     const u32 E = fi == 0 ? 2u : 2u * F.e[fi - 1];
     const u32 pm = small_mod(pr, r, mg, c32);
-    // pm is a unit mod r on every rung that is not skipped: find its inverse.
-    u32 pinv = 0;
-    const u32 one = r == 1 ? 0u : 1u;
-    for (u32 x = lane; x < r; x += 64) if (mod32(pm * x, r, mg) == one) pinv = x + 1;
-    const u64 bal = __ballot(pinv != 0);
-    pinv = bal ? __shfl(pinv, __ffsll((long long)bal) - 1, 64) - 1 : 0;
+    const u32 pinv = unit_inverse(pm, r, mg, lane);
     u32 s[4];
     u128 acc[4];
 #pragma unroll
@@ -862,7 +901,7 @@ __device__ bool help_split(RhoShare *rs, u32 &seen, u32 wv, u32 lane) {
   const u64 n = __builtin_amdgcn_readfirstlane((u32)v->n) |
                 ((u64)__builtin_amdgcn_readfirstlane((u32)(v->n >> 32)) << 32);
   if (__builtin_amdgcn_readfirstlane(v->gen) == gen && n > 1) {
-    const RhoRole ro{rs, 1u, gen, 128u * (wv + 1)};
+    const RhoRole ro{rs, 1u, gen, 64u * kRhoWalks * (wv + 1)};   // disjoint from every other wave's walks
     (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
   }
   return true;
@@ -871,7 +910,7 @@ __device__ bool help_split(RhoShare *rs, u32 &seen, u32 wv, u32 lane) {
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
-                                                          TrialTab T, RungOut *__restrict__ out) {
+                                                          TrialTab T, RungOut *__restrict__ out, u32 *queue) {
   __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -880,10 +919,18 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   __shared__ RungOut wres[kRungWaves];
   __shared__ RhoShare rs;
   __shared__ u32 mr_composite;
+  __shared__ u32 cur_idx[2];
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
-  for (u32 idx = blockIdx.x; idx < n_; idx += gridDim.x) {
+  // A persistent grid takes survivors from a queue, so every prime starts as soon as a
+  // workgroup is free (the index slot alternates, so the next claim never overwrites an
+  // index a wave has yet to read).
+  for (u32 it = 0;; ++it) {
+    if (tid == 0) cur_idx[it & 1] = atomicAdd(queue, 1u);
+    __syncthreads();
+    const u32 idx = cur_idx[it & 1];
+    if (idx >= n_) break;
     const u8 f0 = flags[idx];
     if (f0 == kComposite) continue;
     const u64 p = vals[idx];
@@ -1097,7 +1144,7 @@ struct DeviceCtx::Impl {
   dev::RungOut *d_rung = nullptr;   // stage D result per survivor
   dev::TrialTab trial{};            // odd primes below 2^16 with their inverses
   size_t surv_cap = 0;
-  u32 *d_scalars = nullptr;   // [0] survivor count, [1] queue head
+  u32 *d_scalars = nullptr;   // [0] survivor count, [1] tile queue head, [2] stage-D queue head
   u32 *h_count = nullptr;
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
   // writes the results into (device-side addresses in the d_h* twins).
@@ -1365,7 +1412,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
   // The device part of one batch: reset, sieve, Miller-Rabin. The persistent grid does
   // not depend on the batch, which is what lets a graph of it be replayed.
   auto enqueue = [&](bool timed) -> int {
-    dev::k_reset<<<1, 64, 0, m.stream>>>(m.d_scalars, m.d_scalars + 1);
+    dev::k_reset<<<1, 64, 0, m.stream>>>(m.d_scalars);
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(m.ev1, m.stream));
     if (cfg.persistent) {
@@ -1384,8 +1431,8 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
     HIP_TRY(hipGetLastError());
     if (cfg.device_tail) {
       // k_rung finishes the Miller-Rabin bases of the pending values itself.
-      dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(
-          m.d_surv_p, m.d_scalars, (u32)m.surv_cap, m.d_flags, m.d_one, m.trial, m.d_rung);
+      dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, m.stream>>>(
+          m.d_surv_p, m.d_scalars, (u32)m.surv_cap, m.d_flags, m.d_one, m.trial, m.d_rung, m.d_scalars + 2);
     } else {
       dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
                                                                     m.d_flags, m.d_one);
@@ -1469,14 +1516,15 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   dev::RungOut *d_o = nullptr;
   HIP_TRY(hipMalloc(&d_v, n * 8));
   HIP_TRY(hipMalloc(&d_f, n));
-  HIP_TRY(hipMalloc(&d_n, 4));
+  HIP_TRY(hipMalloc(&d_n, 8));
   HIP_TRY(hipMalloc(&d_o, n * sizeof(dev::RungOut)));
-  const u32 n32 = (u32)n;
+  const u32 n32[2] = {(u32)n, 0u};   // count, queue head
   std::vector<uint8_t> ones(n, 1);
   HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_f, ones.data(), n, hipMemcpyHostToDevice, m.stream));
-  HIP_TRY(hipMemcpyAsync(d_n, &n32, 4, hipMemcpyHostToDevice, m.stream));
-  dev::k_rung<<<std::max(1, cus_ * 4), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32, d_f, nullptr, m.trial, d_o);
+  HIP_TRY(hipMemcpyAsync(d_n, n32, 8, hipMemcpyHostToDevice, m.stream));
+  dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32[0], d_f, nullptr, m.trial, d_o,
+                                                                         d_n + 1);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
