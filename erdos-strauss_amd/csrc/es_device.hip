@@ -645,7 +645,7 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
 }
 
 
-constexpr int kRhoWalks = 8;   // independent walks per lane: FP64 latency is ~8 issue slots
+constexpr int kRhoWalks = 4;   // independent walks per lane (4 measured best of 2, 4, 8 on C2)
 
 // wave_rho for n < 2^50 (same contract): 64 x kRhoWalks walks x -> x^2 + c with
 // c = cbase + lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts
