@@ -155,6 +155,50 @@ def c4_bench(es, reps=5):
             "by_wheel": res}
 
 
+def tuned_table_bench(es, block, bits, steps):
+    """The MI355X-tuned filter table (tables/filters/f_M2042040_mmax2000_mi355x.txt, made by
+    bin/es_prune) beside the reference's GPU table: the 1e15 window [1e15, 1e15+2^34) as a
+    graph step (record compared with the reference scanner's record for that table every
+    step), and a 32-window batch of consecutive 2^34 windows from 1e15 (throughput; the
+    first window's record compared). Same launch shape for both tables."""
+    gold = open(os.path.join(REPO, "tests", "golden", "records", "c2_mi355x.txt")).read()
+    wins = [(LO + i * (HI - LO), LO + (i + 1) * (HI - LO)) for i in range(32)]
+    out = {"table": os.path.relpath(es.F204_MI355X, REPO)}
+    for line in open(es.F204_MI355X):
+        if line.startswith("# sha256="):
+            out["sha256"] = line.split("=", 1)[1].strip()
+        elif line.startswith("# filters="):
+            out["table_stats"] = dict(kv.split("=") for kv in line[2:].split())
+    for name, F in (("mi355x", es.F204_MI355X), ("gpu", es.F204)):
+        sc = es.Scanner(wheel=2042040, filters=F, device=0)
+        sc.set_launch(True, block, bits)
+        sc.set_graph(True)
+        r = {}
+        if name == "mi355x":
+            for _ in range(3):
+                sc.records([(LO, HI)])
+            ok = True
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                (rec,), _ = sc.records([(LO, HI)])
+                ok = ok and rec == gold
+            wall = time.perf_counter() - t0
+            r["c2"] = {"value": (HI - LO) * steps / wall, "ms_per_step": wall * 1e3 / steps, "record_identical": ok,
+                       "device_ms": sc.timing()["device_ms"]}
+        recs, _ = sc.records(wins)
+        first_ok = name != "mi355x" or recs[0] == gold
+        t0 = time.perf_counter()
+        for _ in range(5):
+            sc.records(wins)
+        wall = (time.perf_counter() - t0) / 5
+        r["batch32"] = {"value": (wins[-1][1] - wins[0][0]) / wall, "ms_per_batch": wall * 1e3,
+                        "device_ms": sc.timing()["device_ms"], "first_record_identical": first_ok}
+        out[name] = r
+        sc.close()
+    out["batch32_over_gpu_table"] = out["mi355x"]["batch32"]["value"] / out["gpu"]["batch32"]["value"]
+    return out
+
+
 def c5_bench(es, seconds):
     """Serving configuration: the Poisson request trace (200 req/s, seed 1, start
     log-uniform over [1e6, 1.8e19], length log-uniform over [1e4, 1e10]) replayed in real
@@ -193,6 +237,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched 512-window configuration")
+    ap.add_argument("--no-tuned", action="store_true", help="skip the MI355X-tuned filter table leg")
     ap.add_argument("--c5-seconds", type=float, default=10.0,
                     help="seconds of the serving trace to replay in real time (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -305,6 +350,8 @@ def main():
                      "traffic_source": "profiles/k_sieve_c2_hbm_traffic.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE "
                                        "per k_sieve dispatch on this workload, scaled by a 1 GiB copy"},
     }
+    if not args.no_tuned:
+        out["tuned_table"] = tuned_table_bench(es, block, bits, args.steps)
     if not args.no_c4:
         out["c4"] = c4_bench(es)
     if args.c5_seconds > 0:

@@ -13,6 +13,7 @@ BIN_PATH = os.path.join(HERE, "bin", "es_sieve")
 FILTERS = os.path.join(HERE, "tables", "filters")
 F120 = os.path.join(FILTERS, "f_M120120_mmax2000_runtime.txt")
 F204 = os.path.join(FILTERS, "f_M2042040_mmax2000_gpu.txt")
+F204_MI355X = os.path.join(FILTERS, "f_M2042040_mmax2000_mi355x.txt")   # chosen by bin/es_prune
 F204R = os.path.join(FILTERS, "f_M2042040_mmax2000_runtime.txt")
 
 U64 = ctypes.c_uint64

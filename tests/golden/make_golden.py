@@ -20,7 +20,8 @@ What it writes:
   c5/records_first400.txt  records of the trace's first 400 requests (--c5-only)
   shards/lane/shard_*.meta shard metas of a 7-shard run of the reference binary (--shards-only)
 
-Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only | --shards-only]
+Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only | --shards-only |
+                                            --only NAME[,NAME...]]
   --skip-slow  leave out the top-of-range interval, whose reference startup
                (a base-prime sieve to sqrt(hi)) takes minutes and ~9 GB.
 """
@@ -41,6 +42,7 @@ FILT = os.path.join(REPO, "erdos-strauss_amd", "tables", "filters")
 F120 = os.path.join(FILT, "f_M120120_mmax2000_runtime.txt")
 F204 = os.path.join(FILT, "f_M2042040_mmax2000_gpu.txt")
 F204R = os.path.join(FILT, "f_M2042040_mmax2000_runtime.txt")
+F204M = os.path.join(FILT, "f_M2042040_mmax2000_mi355x.txt")   # the MI355X-tuned table (es_prune)
 DIRECT_PMAX = 18446744065119614976
 TOP = DIRECT_PMAX
 
@@ -72,6 +74,12 @@ RECORDS = [
     ("mmax60_w120_nosurv", 10**9, 10**9 + 10**8, ["--wheel", "120120", "--mmax", "60", "--no-emit-survivors"]),
     # above 2^63 at moderate cost
     ("above63", 2**63 + 10**9, 2**63 + 10**9 + 4 * 10**9, ["--filters", F204]),
+    # the MI355X-tuned table, read by the reference scanner itself
+    ("c2_mi355x", 10**15, 10**15 + 2**34, ["--wheel", "2042040", "--filters", F204M]),
+    ("d12_mi355x", 10**12, 10**12 + 10**10, ["--filters", F204M]),
+    ("d18_mi355x", 10**18, 10**18 + 4 * 10**9, ["--filters", F204M]),
+    ("above63_mi355x", 2**63 + 10**9, 2**63 + 10**9 + 4 * 10**9, ["--filters", F204M]),
+    ("top_mi355x", TOP - 2**30, TOP, ["--filters", F204M]),
 ]
 SLOW = {"c3"}
 
@@ -244,6 +252,12 @@ def main():
         return
     if "--c5-only" in sys.argv:
         c5_records()
+        return
+    if "--only" in sys.argv:
+        names = set(sys.argv[sys.argv.index("--only") + 1].split(","))
+        for name, lo, hi, extra in RECORDS:
+            if name in names:
+                run_ref(name, lo, hi, extra)
         return
     os.makedirs(os.path.join(HERE, "records"), exist_ok=True)
     c4_windows()

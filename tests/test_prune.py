@@ -1,0 +1,87 @@
+"""The filter-table chooser (bin/es_prune) and the tables it made, on the CPU:
+
+- every shipped table carries the sha256 of its own text (the reference's convention:
+  the digest of the file without its digest line);
+- every FILTER of every shipped table is a certified family member whose modulus does
+  not divide the wheel (F1 cover or F2 uniform rung certificate);
+- given the reference's L4 constants and cost form, es_prune reproduces the reference's
+  published GPU table: the same summary statistics to every printed digit, and the same
+  filters up to ties between filters that kill exactly the same positions;
+- given the MI355X constants, it regenerates the committed MI355X table byte for byte.
+"""
+import hashlib
+import os
+import subprocess
+
+import pytest
+
+from tests import paths
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(paths.PKG, "csrc")
+PRUNE = os.path.join(paths.PKG, "bin", "es_prune")
+TABLES = sorted(os.path.join(paths.FILTERS, f) for f in os.listdir(paths.FILTERS) if f.endswith(".txt"))
+GPU_TABLE = os.path.join(paths.FILTERS, "f_M2042040_mmax2000_gpu.txt")
+MI_TABLE = os.path.join(paths.FILTERS, "f_M2042040_mmax2000_mi355x.txt")
+
+# The constants the MI355X table was made with (its header records them).
+MI_ARGS = ["--cost-form", "ref", "--ns-mark", "0.00001", "--ns-pair", "0.42", "--tile-positions", "2448532",
+           "--ns-comp", "3.5", "--ns-prime", "140", "--tag", "mi355x"]
+# The reference's L4 device constants (prune_filters.py:161-186) and its per-step cost form.
+REF_ARGS = ["--cost-form", "ref", "--ns-mark", "0.004280", "--ns-pair", "7.03", "--tile-positions", "2448532",
+            "--ns-comp", "29.6", "--ns-prime", "2345.6", "--tag", "yes"]
+
+
+def filters(path):
+    return [tuple(l.split()[1:4]) for l in open(path) if l.startswith("FILTER")]
+
+
+def summary(path):
+    return [l for l in open(path) if l.startswith("# filters=")][0]
+
+
+@pytest.mark.parametrize("table", TABLES, ids=[os.path.basename(t) for t in TABLES])
+def test_table_digest(table):
+    lines = open(table).read().splitlines(keepends=True)
+    digest = [l for l in lines if l.startswith("# sha256=")][0].split("=", 1)[1].strip()
+    body = "".join(l for l in lines if not l.startswith("# sha256="))
+    assert hashlib.sha256(body.encode()).hexdigest() == digest
+
+
+@pytest.fixture(scope="module")
+def table_check(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("tc") / "table_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, os.path.join(HERE, "replay", "table_check.cpp"),
+                    os.path.join(CSRC, "es_plan.cpp"), "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("table", [t for t in TABLES if "M2042040" in t], ids=lambda t: os.path.basename(t))
+def test_table_filters_are_certified(table_check, table):
+    out = subprocess.run([table_check, table, "2042040", "2000"], stdout=subprocess.PIPE, check=True)
+    assert out.stdout.decode().strip() == "ok %d" % len(filters(table))
+
+
+def prune(tmp_path, args):
+    assert os.path.exists(PRUNE), "bin/es_prune is not built: run __graft_entry__.build()"
+    out = str(tmp_path / "table.txt")
+    p = subprocess.run([PRUNE] + args + ["--out", out], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       timeout=900)
+    assert p.returncode == 0, p.stderr.decode()
+    return out
+
+
+def test_pruner_reproduces_the_reference_gpu_table(tmp_path):
+    got = prune(tmp_path, REF_ARGS)
+    assert summary(got) == summary(GPU_TABLE)
+    a = {f[:2] for f in filters(got)}
+    b = {f[:2] for f in filters(GPU_TABLE)}
+    assert len(a) == len(b) == 3745
+    # The few that differ are ties: each stands in for a filter of the other table with
+    # the same stride on this wheel (identical kills, hence identical statistics above).
+    assert len(a ^ b) <= 10
+
+
+def test_pruner_regenerates_the_mi355x_table(tmp_path):
+    got = prune(tmp_path, MI_ARGS)
+    assert open(got).read() == open(MI_TABLE).read()
