@@ -3,7 +3,7 @@
 
 Stand-alone counterpart of the reference's sieve/verify_covers.py:246-275 (validity
 rules) and sieve/verify_sample.py:37-94 (independent checks), written from those rules,
-standard library only, so it runs wherever the scanner runs (the GPU box included).
+standard library only, so it runs wherever the scanner runs, with or without the reference.
 
 A line is accepted only if the Erdos-Straus identity it claims holds in exact rationals:
 

@@ -3,8 +3,8 @@
 
 Runs only in the development container, where the reference tree is mounted at
 REF_ROOT (default /root/reference) and its scanner has been built from source by
-`make -C oracle ref` into oracle/_ref/rung_scan3. Nothing here runs on the GPU box:
-the tests read the committed outputs.
+`make -C oracle ref` into oracle/_ref/rung_scan3. Nothing here runs where the tests run
+on a GPU: the tests read the committed outputs.
 
 What it writes:
   records/<name>.txt     complete stdout of the reference scanner (rung_scan3) on a
