@@ -313,6 +313,7 @@ def main():
 
     # ---- roofline of the marking kernel (k_sieve)
     lds_ops = sc.lds_peak()
+    scatter = sc.lds_scatter_peak()
     kernel_s = sorted(sieve_ms)[len(sieve_ms) // 2] * 1e-3
     lds_bytes = 8 * work["marks"] + 8 * (work["bits"] // 32)   # 32-bit RMW per mark; init + scan per word
     achieved = lds_bytes / kernel_s / 1e9
@@ -345,6 +346,9 @@ def main():
                      "peak_measured": lds_ops * 8 / 1e9,
                      "frac_of_measured": achieved / (lds_ops * 8 / 1e9),
                      "marks": work["marks"], "bits": work["bits"], "pairs": work["pairs"],
+                     "marks_per_s": work["marks"] / kernel_s,
+                     "scattered_byte_stores_per_s": scatter,
+                     "frac_of_scattered_store_rate": work["marks"] / kernel_s / scatter,
                      "algorithmic_bytes": lds_bytes, "kernel_ms": kernel_s * 1e3,
                      "traffic": profiled_traffic(),
                      "traffic_source": "profiles/k_sieve_c2_hbm_traffic.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE "

@@ -279,6 +279,12 @@ int es_lds_peak(es_scanner *s, double *ops_per_s) {
   return ES_OK;
 }
 
+int es_lds_scatter_peak(es_scanner *s, double *stores_per_s) {
+  if (!s || !stores_per_s) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.device().lds_scatter_peak(*stores_per_s)) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  return ES_OK;
+}
+
 int es_wheel_classes(const es_scanner *s, uint64_t *out, size_t cap, size_t *n) {
   if (!s || !n) return fail(ES_ERR_ARG, "null argument");
   const std::vector<es::u64> &c = s->sc.plan().wheel.classes;

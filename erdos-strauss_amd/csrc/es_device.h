@@ -79,6 +79,7 @@ class DeviceCtx {
   int rung(const u64 *vals, size_t n, RungResult *out);
   // Measured LDS atomic-AND rate of the whole device (32-bit operations per second).
   int lds_peak(double &ops_per_s);
+  int lds_scatter_peak(double &stores_per_s);   // scattered byte stores per second
   const std::string &last_error() const { return err_; }
   int device() const { return device_; }
   size_t layout_bytes() const { return layout_bytes_; }

@@ -1,18 +1,23 @@
 // es_device.hip -- gfx950 kernels of the verification-sieve funnel and the host code
 // that drives them.
 //
-//   k_sieve   one tile = one LDS bitmap of <= max_bits positions of one lane. Init ANDs
-//             the lane's periodic wheel mask (sub-wheel layouts) and the tail mask, and
-//             applies strides < 32 as whole-word patterns (each word owned by one
-//             thread, no atomics). Larger strides clear bits with LDS atomic AND. Stage
-//             B (certified filters) and stage B' (small primes) are separate phases so
-//             the popcount between them splits `covered` from `sieved` exactly
+//   k_sieve   one tile = one LDS byte map of <= max_bits positions of one lane. Init
+//             expands the lane's periodic wheel mask (sub-wheel layouts) and the tail
+//             mask to one byte per position; every filter then clears its positions
+//             with plain byte stores of 0 (idempotent, no atomics). Stage B (certified
+//             filters) and stage B' (small primes) are separate phases so the count
+//             between them splits `covered` from `sieved` exactly
 //             (rung_scan3.cpp:755-772). Survivors are counted per thread, scanned
 //             across the wave with shuffles and across the workgroup in LDS, placed
 //             with one global atomic per tile, and written as 64-bit integers.
 //             Persistent form: workgroups pull tiles from a device queue.
-//   k_mr      deterministic Miller-Rabin (first twelve prime bases) per survivor, in
-//             64-bit Montgomery arithmetic built from __umul64hi.
+//   k_mr_base2  trial prologue and the base-2 strong test per survivor (FP64 residues
+//             below 2^50, 64-bit Montgomery above); k_mr_bases runs the other eleven
+//             bases when stage D stays on the host.
+//   k_rung    stage D on the device: bases 3..37 for pending values, then the rungs
+//             r = 3 (mod 4) up to 255 with a small-prime sieve, least-witness search and
+//             Pollard rho, one persistent workgroup per CU taking survivors from a queue.
+//   k_pack    survivor records and tile counters into pinned host memory.
 //
 // Reference behaviour: sieve/cuda/mark.cu:22-91 (k_mark), survivors.cu:20-133
 // (k_scan_counts, k_compact, k_primality), rung_scan3.cpp:711-784 (sweep_lane).
@@ -1102,6 +1107,25 @@ __global__ __launch_bounds__(256) void k_lds_peak(u32 iters, u32 *sink) {
   if (w[tid] == 0x12345678u) sink[blockIdx.x] = tid;   // never true; keeps the loop live
 }
 
+// LDS rate of the marking primitive itself, in its access pattern: byte stores of 0 in
+// which every lane walks its own odd stride (19 .. 2001, like the filters of one bin) over
+// a 16 KB map, so the lanes of a store land on banks independently of one another;
+// 512-thread workgroups three to a CU, as the marking kernel runs.
+__global__ __launch_bounds__(512) void k_lds_scatter(u32 iters, u32 *sink) {
+  __shared__ u8 map[16384];
+  const u32 tid = threadIdx.x;
+  for (u32 i = tid; i < 16384 / 4; i += blockDim.x) ((u32 *)map)[i] = 0x01010101u;
+  __syncthreads();
+  const u32 s = 19u + 2u * ((tid * 7919u + blockIdx.x * 31u) % 992u);
+  u32 a = (tid * 2654435761u + blockIdx.x * 97u) & 16383u;
+  for (u32 i = 0; i < iters; ++i) {
+    map[a] = 0;
+    a = (a + s) & 16383u;
+  }
+  __syncthreads();
+  if (map[tid] == 7) sink[blockIdx.x] = tid;   // never true; keeps the stores live
+}
+
 }  // namespace dev
 
 // ====================================================================== host side
@@ -1544,6 +1568,27 @@ size_t DeviceCtx::resident_bytes() const {
   const size_t per_surv = sizeof(u64) * 2 + sizeof(u32) + 1 + sizeof(dev::RungOut);
   return layout_bytes_ + m.trial.n * (sizeof(uint2) + 16) + m.tiles_cap * sizeof(TileDesc) +
          m.tiles_cap * sizeof(uint4) + m.surv_cap * per_surv;
+}
+
+int DeviceCtx::lds_scatter_peak(double &stores_per_s) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  u32 *sink = nullptr;
+  const u32 blocks = (u32)cus_ * 3, iters = 1u << 14;
+  HIP_TRY(hipMalloc(&sink, blocks * sizeof(u32)));
+  dev::k_lds_scatter<<<blocks, 512, 0, m.stream>>>(64, sink);   // warm-up
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m.ev0, m.stream));
+  dev::k_lds_scatter<<<blocks, 512, 0, m.stream>>>(iters, sink);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m.ev1, m.stream));
+  HIP_TRY(hipStreamSynchronize(m.stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, m.ev0, m.ev1));
+  HIP_TRY(hipFree(sink));
+  stores_per_s = (double)blocks * 512.0 * iters / (ms * 1e-3);
+  return 0;
 }
 
 int DeviceCtx::lds_peak(double &ops_per_s) {

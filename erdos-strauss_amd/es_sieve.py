@@ -109,6 +109,7 @@ def lib():
             "es_wheel_info": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), P(U64), P(U64)]),
             "es_mark_count": (ctypes.c_int, [ctypes.c_void_p, U64, U64, U64, P(U64), P(U64), P(U64), P(U64)]),
             "es_lds_peak": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
+            "es_lds_scatter_peak": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_double)]),
             "es_wheel_classes": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_size_t)]),
             "es_serve_config_default": (None, [P(EsServeConfig)]),
             "es_server_open": (ctypes.c_int, [ctypes.c_void_p, P(EsServeConfig), P(ctypes.c_void_p)]),
@@ -287,6 +288,12 @@ class Scanner:
         """Measured LDS atomic-AND operations per second."""
         v = ctypes.c_double()
         _check(lib().es_lds_peak(self.h, ctypes.byref(v)))
+        return v.value
+
+    def lds_scatter_peak(self):
+        """Measured LDS scattered byte stores per second (the marking primitive)."""
+        v = ctypes.c_double()
+        _check(lib().es_lds_scatter_peak(self.h, ctypes.byref(v)))
         return v.value
 
     def wheel_classes(self):

@@ -170,6 +170,11 @@ int es_mark_count(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t Mp, uint64_t
  * (sieve/cuda/mark.cu:197-263), which times the marking kernel itself. */
 int es_lds_peak(es_scanner *s, double *ops_per_s);
 
+/* Measured LDS rate of scattered byte stores (stores per second), at the marking kernel's
+ * occupancy: the rate the marking primitive can reach when its addresses share banks at
+ * random. Second denominator for the same --bench-mark comparison (mark.cu:197-263). */
+int es_lds_scatter_peak(es_scanner *s, double *stores_per_s);
+
 /* The wheel classes (ascending), for lane-order merging of window records. Writes
  * min(cap, lanes) values and returns ES_OK; *n receives the lane count. Replaces
  * ref3_wheel (sieve/cuda/ref_host.h:33). */
