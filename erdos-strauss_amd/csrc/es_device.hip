@@ -46,7 +46,7 @@ struct LayoutDev {
   const u32 *c32;
   const uint4 *fc;           // per ordered filter: {s, floor(2^32/s), 2^32 mod s, 0}
   const u16 *off;            // per lane a row of ostride offsets (nf used, 0xFFFF after)
-  u32 P, Pmagic, Pc32, nl, nf, ncov, cov_small, pr_small, mask_words, ostride;
+  u32 P, Pmagic, Pc32, nl, nf, ncov, mask_words, ostride;
 };
 
 struct BatchHdr {
@@ -1309,8 +1309,6 @@ int DeviceCtx::add_layout(const Layout &L) {
   d.nl = (u32)L.lanes.size();
   d.nf = L.nf;
   d.ncov = L.ncov;
-  d.cov_small = L.cov_small;
-  d.pr_small = L.pr_small;
   d.mask_words = L.mask_words;
   d.ostride = ostride;
   m.layouts.push_back(b);

@@ -242,25 +242,29 @@ bool build_layout(u64 M, u64 Mp, const std::vector<u64> &classes, const FilterSe
         L.mask[i * L.mask_words + k / 32] |= 1u << (k % 32);
 
   // Marking order: covers sorted by stride, then primes sorted by stride. The order is
-  // free (clearing is idempotent); sorted strides balance the threads of a wave.
-  struct F { u32 idx; u64 g, s; };
+  // free (clearing is idempotent); sorted strides balance the threads of a wave. A filter
+  // of stride b < kSplitStride is marked as k = ceil(kSplitStride / b) pieces of stride
+  // s = k*b, piece t taking the hits o + t*b (mod s): the same positions, in runs as long
+  // as those of the larger strides they are sorted among. Short strides are few, so a bin
+  // of 64 filters would otherwise mix strides whose run lengths differ by half or more,
+  // and the wave would wait on the longest.
+  struct F { u32 idx, k, t; u64 g, b, s; };
   std::vector<F> cov, pr;
   for (size_t i = 0; i < fs.mod.size(); ++i) {
     const u64 m = fs.mod[i];
     if (m == 0) { msg = "zero filter modulus"; return false; }
-    const u64 g = gcd64(m, Mp), s = m / g;
-    if (s >= 65535) { msg = "filter stride does not fit the 16-bit offset tables"; return false; }
-    (i < fs.ncov ? cov : pr).push_back(F{(u32)i, g, s});
+    const u64 g = gcd64(m, Mp), b = m / g;
+    const u32 k = b < kSplitStride ? (u32)((kSplitStride + b - 1) / b) : 1u;
+    if (k * b >= 65535) { msg = "filter stride does not fit the 16-bit offset tables"; return false; }
+    for (u32 t = 0; t < k; ++t) (i < fs.ncov ? cov : pr).push_back(F{(u32)i, k, t, g, b, k * b});
   }
-  auto by_s = [](const F &a, const F &b) { return a.s != b.s ? a.s < b.s : a.idx < b.idx; };
-  std::stable_sort(cov.begin(), cov.end(), by_s);
-  std::stable_sort(pr.begin(), pr.end(), by_s);
+  auto by_s = [](const F &a, const F &b) { return a.s != b.s ? a.s < b.s : a.idx != b.idx ? a.idx < b.idx : a.t < b.t; };
+  std::sort(cov.begin(), cov.end(), by_s);
+  std::sort(pr.begin(), pr.end(), by_s);
   std::vector<F> all = cov;
   all.insert(all.end(), pr.begin(), pr.end());
   L.nf = (u32)all.size();
   L.ncov = (u32)cov.size();
-  for (const F &f : cov) L.cov_small += f.s < kWordWiseStep;
-  for (const F &f : pr) L.pr_small += f.s < kWordWiseStep;
   L.order.resize(L.nf);
   L.step.resize(L.nf);
   L.magic.resize(L.nf);
@@ -272,7 +276,7 @@ bool build_layout(u64 M, u64 Mp, const std::vector<u64> &classes, const FilterSe
     L.step[j] = (u32)f.s;
     L.magic[j] = f.s == 1 ? 0xFFFFFFFFu : (u32)((1ull << 32) / f.s);
     L.c32[j] = (u32)((1ull << 32) % f.s);
-    if (!inverse_mod((Mp / f.g) % f.s, f.s, inv[j])) { msg = "non-invertible layout stride"; return false; }
+    if (!inverse_mod((Mp / f.g) % f.b, f.b, inv[j])) { msg = "non-invertible layout stride"; return false; }
   }
   L.off.assign(nl * (size_t)L.nf, kNoHit);
   double marks = 0;
@@ -283,7 +287,8 @@ bool build_layout(u64 M, u64 Mp, const std::vector<u64> &classes, const FilterSe
       const u64 m = fs.mod[f.idx];
       const u64 rhs = (fs.res[f.idx] % m + m - c % m) % m;   // Mp*k == res - c' (mod m)
       if (rhs % f.g) continue;
-      L.off[i * L.nf + j] = f.s == 1 ? 0 : (u16)((u128)(rhs / f.g) * inv[j] % f.s);
+      const u64 o = f.b == 1 ? 0 : (u64)((u128)(rhs / f.g) * inv[j] % f.b);
+      L.off[i * L.nf + j] = (u16)(o + f.t * f.b);
       marks += 1.0 / (double)f.s;
       ++L.hit_pairs;
     }

@@ -60,27 +60,27 @@ void finish_filter_digest(FilterSet &fs);                                       
 // hard classes. Wheel membership of p depends on k mod P (P = M/Mp) only, so each lane
 // carries a periodic P-bit mask. A filter (m, res) hits lane c' at k == o (mod s) with
 // g = gcd(m, Mp), s = m/g, o = ((res - c')/g) * inv(Mp/g mod s, s) mod s, iff
-// g | (res - c'); o depends on (lane, filter) only, never on the window.
+// g | (res - c'); o depends on (lane, filter) only, never on the window. A filter with
+// s < kSplitStride is marked as k pieces of stride k*s (piece t: offset o + t*s).
 struct Layout {
   u64 Mp = 0, P = 1;
   std::vector<u64> lanes;             // c' ascending
   u32 mask_words = 0;                 // per lane: the P-bit pattern extended by 64 bits
   std::vector<u32> mask;              // [lanes][mask_words]
-  // Filters in marking order: covers by ascending s, then small primes by ascending s.
+  // Marking units (filters, or pieces of split filters) in marking order: covers by
+  // ascending stride, then small primes by ascending stride.
   std::vector<u32> order;             // index into FilterSet
-  std::vector<u32> step;              // s per ordered filter
+  std::vector<u32> step;              // stride s of each unit
   std::vector<u32> magic;             // floor(2^32 / s) (2^32-1 for s = 1)
   std::vector<u32> c32;               // 2^32 mod s
-  u32 nf = 0, ncov = 0;               // ordered filters; covers among them
-  u32 cov_small = 0;                  // covers with s < 32 (applied word-wise)
-  u32 pr_small = 0;                   // primes with s < 32 (applied word-wise)
+  u32 nf = 0, ncov = 0;               // marking units; covers among them
   std::vector<u16> off;               // [lanes][nf]: o, or kNoHit
   double marks_per_int = 0;           // expected marks per integer of [lo, hi)
   double bits_per_int = 0;            // bitmap bits per integer
   u64 hit_pairs = 0;
 };
 constexpr u16 kNoHit = 0xFFFF;
-constexpr u32 kWordWiseStep = 32;     // strides below this are applied as word patterns
+constexpr u64 kSplitStride = 64;      // shorter strides are marked as pieces of about this stride
 
 // Builds a layout; false (with msg) if a stride or P does not fit the 16-bit tables.
 bool build_layout(u64 M, u64 Mp, const std::vector<u64> &classes, const FilterSet &fs,
