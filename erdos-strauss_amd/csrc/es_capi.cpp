@@ -70,6 +70,7 @@ int es_check_interval(const es_scanner *s, uint64_t lo, uint64_t hi) {
 static int run_windows(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin,
                        std::vector<es::WindowRecord> &recs) {
   if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.in_flight()) return fail(ES_ERR_ARG, "submitted batches are in flight: collect them first");
   std::vector<es::Interval> w(nwin);
   std::string msg;
   for (size_t i = 0; i < nwin; ++i) {
@@ -81,13 +82,10 @@ static int run_windows(es_scanner *s, const uint64_t *lo, const uint64_t *hi, si
   return ES_OK;
 }
 
-int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, int emit_survivors,
-                    char **text, size_t *text_len, size_t *rec_len, int *exit_code) {
-  if (!text) return fail(ES_ERR_ARG, "null text");
-  *text = nullptr;
-  std::vector<es::WindowRecord> recs;
-  const int rc = run_windows(s, lo, hi, nwin, recs);
-  if (rc) return rc;
+// The records of one batch as one malloc'd text, with each record's length and exit code.
+static int records_text(es_scanner *s, const std::vector<es::WindowRecord> &recs, int emit_survivors, char **text,
+                        size_t *text_len, size_t *rec_len, int *exit_code) {
+  const size_t nwin = recs.size();
   std::string all;
   for (size_t i = 0; i < nwin; ++i) {
     const std::string r = s->sc.record_text(recs[i], emit_survivors != 0);
@@ -104,6 +102,43 @@ int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_
   return ES_OK;
 }
 
+int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, int emit_survivors,
+                    char **text, size_t *text_len, size_t *rec_len, int *exit_code) {
+  if (!text) return fail(ES_ERR_ARG, "null text");
+  *text = nullptr;
+  std::vector<es::WindowRecord> recs;
+  const int rc = run_windows(s, lo, hi, nwin, recs);
+  if (rc) return rc;
+  return records_text(s, recs, emit_survivors, text, text_len, rec_len, exit_code);
+}
+
+int es_scan_submit(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin) {
+  if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.in_flight() >= 2) return fail(ES_ERR_ARG, "two batches are already in flight: collect one first");
+  std::vector<es::Interval> w(nwin);
+  std::string msg;
+  for (size_t i = 0; i < nwin; ++i) {
+    w[i] = es::Interval{lo[i], hi[i]};
+    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
+  }
+  if (s->sc.submit(w, msg)) return fail(ES_ERR_DEVICE, msg);
+  return ES_OK;
+}
+
+int es_scan_collect(es_scanner *s, int emit_survivors, char **text, size_t *text_len, size_t *rec_len,
+                    int *exit_code, size_t *nwin) {
+  if (!s || !text) return fail(ES_ERR_ARG, "null argument");
+  *text = nullptr;
+  if (s->sc.in_flight() == 0) return fail(ES_ERR_ARG, "no batch in flight");
+  std::vector<es::WindowRecord> recs;
+  std::string msg;
+  if (s->sc.collect(recs, s->last, msg)) return fail(ES_ERR_DEVICE, msg);
+  if (nwin) *nwin = recs.size();
+  return records_text(s, recs, emit_survivors, text, text_len, rec_len, exit_code);
+}
+
+int es_scan_in_flight(const es_scanner *s) { return s ? (int)s->sc.in_flight() : 0; }
+
 int es_scan_counts(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, es_counts *out) {
   if (!out && nwin) return fail(ES_ERR_ARG, "null output");
   std::vector<es::WindowRecord> recs;
@@ -119,6 +154,7 @@ int es_scan_counts(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t
 int es_survivors(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t *p, uint8_t *is_prime, size_t cap,
                  size_t *n, es_counts *counts) {
   if (!s || !n) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.in_flight()) return fail(ES_ERR_ARG, "submitted batches are in flight: collect them first");
   std::vector<es::Interval> w{es::Interval{lo, hi}};
   std::vector<es::u64> sp;
   std::vector<uint8_t> fl;
@@ -222,6 +258,7 @@ int es_set_tuning(es_scanner *s, double c_pair, double c_mark, double c_bit) {
 
 int es_autotune(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, char **report) {
   if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
+  if (s->sc.in_flight()) return fail(ES_ERR_ARG, "submitted batches are in flight: collect them first");
   std::vector<es::Interval> w(nwin);
   std::string msg, rep;
   for (size_t i = 0; i < nwin; ++i) {

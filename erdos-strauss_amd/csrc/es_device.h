@@ -70,9 +70,16 @@ class DeviceCtx {
   int open(int device);
   // Uploads a layout's tables; returns its id (>= 0) or -1.
   int add_layout(const Layout &L);
-  // Runs one batch. Tiles must reference valid layouts/lanes, nbits <= cfg.max_bits.
+  // Runs one batch (launch then finish; no batch may be in flight). Tiles must reference
+  // valid layouts/lanes, nbits <= cfg.max_bits.
   int run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
           BatchResult &out);
+  // Pipelined form: launch enqueues a batch and returns at once (at most two in flight);
+  // finish waits for the oldest batch in flight and returns its results. The host can
+  // finish one batch while the device runs the next.
+  int launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg);
+  int finish(BatchResult &out);
+  int in_flight() const;
   // Deterministic MR of arbitrary values on the device (parity checks).
   int primality(const u64 *vals, size_t n, uint8_t *flags);
   // Stage D on the device for primes vals[0..n) (one RungResult each).
@@ -92,6 +99,8 @@ class DeviceCtx {
  private:
   struct Impl;
   Impl *impl_ = nullptr;
+  int grow_surv(int slot, size_t need);
+  int enqueue_batch(int slot);
   std::string err_;
   int device_ = -1;
   int cus_ = 0;

@@ -1145,17 +1145,18 @@ struct DevLayoutBufs {
        *fc = nullptr, *off = nullptr;
 };
 
-struct DeviceCtx::Impl {
-  hipStream_t stream = nullptr;
+// One batch's buffers, captured graph and events. Two slots let the host plan and launch
+// batch i+1 while the device still runs batch i, then finish batch i (mirrors, stage-D
+// leftovers, records) while batch i+1 runs: the overlap of the reference driver
+// (sieve/cuda/scan.cu:1-6, 197-207), over whole batches.
+struct Slot {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_ok = false;
-  u32 graph_block = 0, graph_bits = 0;
+  u32 graph_block = 0, graph_grid = 0;
+  size_t graph_lds = 0;
   bool graph_tail = false;
-  std::vector<DevLayoutBufs> layouts;
-  dev::LayoutDev *d_lays = nullptr;
-  size_t lays_cap = 0;
   TileDesc *d_tiles = nullptr;
   size_t tiles_cap = 0;
   dev::BatchHdr *d_hdr = nullptr;
@@ -1164,14 +1165,13 @@ struct DeviceCtx::Impl {
   u64 *d_surv_p = nullptr;
   u32 *d_surv_w = nullptr;
   uint8_t *d_flags = nullptr;
-  u64 *d_one = nullptr;       // 2^64 mod n per survivor, from the base-2 pass
+  u64 *d_one = nullptr;             // 2^64 mod n per survivor, from the base-2 pass
   dev::RungOut *d_rung = nullptr;   // stage D result per survivor
-  dev::TrialTab trial{};            // odd primes below 2^16 with their inverses
   size_t surv_cap = 0;
   u32 *d_scalars = nullptr;   // [0] survivor count, [1] tile queue head, [2] stage-D queue head
-  u32 *h_count = nullptr;
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
-  // writes the results into (device-side addresses in the d_h* twins).
+  // writes the results into (device-side addresses in the dh_* twins).
+  u32 *h_count = nullptr;
   TileDesc *h_tiles = nullptr;
   uint4 *h_tile_counts = nullptr;
   u64 *h_surv_p = nullptr;
@@ -1184,9 +1184,29 @@ struct DeviceCtx::Impl {
   u32 *dh_surv_w = nullptr;
   uint8_t *dh_flags = nullptr;
   dev::RungOut *dh_rung = nullptr;
+  // The batch the slot holds while it is in flight.
+  size_t nt = 0;
+  u32 nwin = 0;
+  LaunchCfg cfg;
+  size_t lds = 0;
+  u32 lds_words = 0, off_words = 0, grid = 0;
+};
+
+struct DeviceCtx::Impl {
+  hipStream_t stream = nullptr;     // every batch, in submission order
+  hipStream_t capture = nullptr;    // graph capture only: nothing ever runs on it
+  hipEvent_t pev0 = nullptr, pev1 = nullptr;   // the LDS probes' timing
+  std::vector<DevLayoutBufs> layouts;
+  dev::LayoutDev *d_lays = nullptr;
+  size_t lays_cap = 0;
+  dev::TrialTab trial{};            // odd primes below 2^16 with their inverses
   int occupancy_blocks = 0;
-  u32 occ_block = 0, occ_bits = 0;
+  u32 occ_block = 0;
+  size_t occ_lds = 0;
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
+  Slot slot[2];
+  int head = 0;                     // the oldest batch in flight
+  int busy = 0;                     // batches in flight (0, 1 or 2)
 };
 
 DeviceCtx::DeviceCtx() : impl_(new Impl) {}
@@ -1199,17 +1219,22 @@ DeviceCtx::~DeviceCtx() {
     if (m.stream) (void)hipStreamSynchronize(m.stream);
     for (DevLayoutBufs &b : m.layouts)
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
-    for (void *p : {(void *)m.d_lays, (void *)m.d_tiles, (void *)m.d_hdr, (void *)m.d_tile_counts,
-                    (void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one,
-                    (void *)m.d_rung, (void *)m.d_scalars, (void *)m.trial.qm, (void *)m.trial.inv,
-                    (void *)m.trial.lim})
+    for (void *p : {(void *)m.d_lays, (void *)m.trial.qm, (void *)m.trial.inv, (void *)m.trial.lim})
       if (p) (void)hipFree(p);
-    for (void *p : {(void *)m.h_hdr, (void *)m.h_count, (void *)m.h_tiles, (void *)m.h_tile_counts,
-                    (void *)m.h_surv_p, (void *)m.h_surv_w, (void *)m.h_flags, (void *)m.h_rung})
-      if (p) (void)hipHostFree(p);
-    if (m.graph_exec) (void)hipGraphExecDestroy(m.graph_exec);
-    if (m.graph) (void)hipGraphDestroy(m.graph);
-    for (hipEvent_t e : {m.ev0, m.ev1, m.ev2, m.ev3}) if (e) (void)hipEventDestroy(e);
+    for (Slot &b : m.slot) {
+      for (void *p : {(void *)b.d_tiles, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
+                      (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung,
+                      (void *)b.d_scalars})
+        if (p) (void)hipFree(p);
+      for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_tiles, (void *)b.h_tile_counts,
+                      (void *)b.h_surv_p, (void *)b.h_surv_w, (void *)b.h_flags, (void *)b.h_rung})
+        if (p) (void)hipHostFree(p);
+      if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+      if (b.graph) (void)hipGraphDestroy(b.graph);
+      for (hipEvent_t e : {b.ev0, b.ev1, b.ev2, b.ev3}) if (e) (void)hipEventDestroy(e);
+    }
+    for (hipEvent_t e : {m.pev0, m.pev1}) if (e) (void)hipEventDestroy(e);
+    if (m.capture) (void)hipStreamDestroy(m.capture);
     if (m.stream) (void)hipStreamDestroy(m.stream);
   }
   delete impl_;
@@ -1233,15 +1258,20 @@ int DeviceCtx::open(int device) {
   lds_limit_ = p.sharedMemPerBlock;
   Impl &m = *impl_;
   HIP_TRY(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&m.ev0));
-  HIP_TRY(hipEventCreate(&m.ev1));
-  HIP_TRY(hipEventCreate(&m.ev2));
-  HIP_TRY(hipEventCreate(&m.ev3));
-  HIP_TRY(hipMalloc(&m.d_hdr, sizeof(dev::BatchHdr)));
-  HIP_TRY(hipHostMalloc(&m.h_hdr, sizeof(dev::BatchHdr)));
-  HIP_TRY(hipMalloc(&m.d_scalars, 4 * sizeof(u32)));
-  HIP_TRY(hipHostMalloc(&m.h_count, 4 * sizeof(u32), hipHostMallocMapped));
-  HIP_TRY(hipHostGetDevicePointer((void **)&m.dh_count, m.h_count, 0));
+  HIP_TRY(hipStreamCreateWithFlags(&m.capture, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&m.pev0));
+  HIP_TRY(hipEventCreate(&m.pev1));
+  for (Slot &b : m.slot) {
+    HIP_TRY(hipEventCreate(&b.ev0));
+    HIP_TRY(hipEventCreate(&b.ev1));
+    HIP_TRY(hipEventCreate(&b.ev2));
+    HIP_TRY(hipEventCreate(&b.ev3));
+    HIP_TRY(hipMalloc(&b.d_hdr, sizeof(dev::BatchHdr)));
+    HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr)));
+    HIP_TRY(hipMalloc(&b.d_scalars, 4 * sizeof(u32)));
+    HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
+    HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
+  }
   // Sieve table for stage D: odd primes q below 2^16 packed as {q | (2^32 mod q) << 16,
   // floor(2^32/q)} (residues of A by q), and q^{-1} mod 2^64, floor((2^64-1)/q) (exact
   // division by q).
@@ -1312,7 +1342,10 @@ int DeviceCtx::add_layout(const Layout &L) {
   d.mask_words = L.mask_words;
   d.ostride = ostride;
   m.layouts.push_back(b);
-  if (ostride > m.off_max) { m.off_max = ostride; m.graph_ok = false; }
+  if (ostride > m.off_max) {
+    m.off_max = ostride;
+    for (Slot &b : m.slot) b.graph_ok = false;
+  }
   // Mirror the descriptor array on the device.
   if (m.layouts.size() > m.lays_cap) {
     if (m.d_lays) HIP_TRY(hipFree(m.d_lays));
@@ -1325,19 +1358,120 @@ int DeviceCtx::add_layout(const Layout &L) {
   return (int)m.layouts.size() - 1;
 }
 
-int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
-                   BatchResult &out) {
+// Pinned host buffer with its device-side address (for k_pack's stores).
+static hipError_t pinned_alloc(void **h, void **d, size_t bytes) {
+  if (*h) {
+    const hipError_t e = hipHostFree(*h);
+    if (e != hipSuccess) return e;
+  }
+  *h = nullptr;
+  hipError_t e = hipHostMalloc(h, bytes, hipHostMallocMapped);
+  if (e == hipSuccess && d) e = hipHostGetDevicePointer(d, *h, 0);
+  return e;
+}
+
+// Survivor buffers of slot si, grown to hold need entries. Any reallocation invalidates
+// the slot's captured graph.
+int DeviceCtx::grow_surv(int si, size_t need) {
+  Slot &b = impl_->slot[si];
+  if (need <= b.surv_cap) return 0;
+  size_t cap = std::max<size_t>(b.surv_cap ? b.surv_cap : (1u << 16), 1u << 16);
+  while (cap < need) cap *= 2;
+  if (cap > (1ull << 28)) { err_ = "survivor count exceeds 2^28 per batch"; return -1; }
+  for (void *p : {(void *)b.d_surv_p, (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung})
+    if (p) HIP_TRY(hipFree(p));
+  HIP_TRY(hipMalloc(&b.d_surv_p, cap * sizeof(u64)));
+  HIP_TRY(hipMalloc(&b.d_one, cap * sizeof(u64)));
+  HIP_TRY(hipMalloc(&b.d_rung, cap * sizeof(dev::RungOut)));
+  HIP_TRY(hipMalloc(&b.d_surv_w, cap * sizeof(u32)));
+  HIP_TRY(hipMalloc(&b.d_flags, cap));
+  HIP_TRY(pinned_alloc((void **)&b.h_surv_p, (void **)&b.dh_surv_p, cap * sizeof(u64)));
+  HIP_TRY(pinned_alloc((void **)&b.h_surv_w, (void **)&b.dh_surv_w, cap * sizeof(u32)));
+  HIP_TRY(pinned_alloc((void **)&b.h_flags, (void **)&b.dh_flags, cap));
+  HIP_TRY(pinned_alloc((void **)&b.h_rung, (void **)&b.dh_rung, cap * sizeof(dev::RungOut)));
+  b.surv_cap = cap;
+  b.graph_ok = false;
+  return 0;
+}
+
+// Enqueues slot si's batch on the stream: reset, sieve, Miller-Rabin, stage D and pack,
+// between events ev0 and ev3 (ev1/ev2 bracket the sieve kernel when it is not replayed
+// from a graph). With the graph on, the batch is captured once per slot and launch shape
+// (on a stream where nothing runs) and replayed after that. The persistent grid does not
+// depend on the batch, which is what lets one graph serve every batch that fits.
+int DeviceCtx::enqueue_batch(int si) {
+  Impl &m = *impl_;
+  Slot &b = m.slot[si];
+  const LaunchCfg &cfg = b.cfg;
+  auto enqueue = [&](hipStream_t st, bool timed) -> int {
+    dev::k_reset<<<1, 64, 0, st>>>(b.d_scalars);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(b.ev1, st));
+    if (cfg.persistent) {
+      dev::k_sieve<true><<<b.grid, cfg.block, b.lds, st>>>(b.d_tiles, b.d_hdr, m.d_lays, b.d_tile_counts,
+                                                           b.d_surv_p, b.d_surv_w, (u32)b.surv_cap,
+                                                           b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
+    } else {
+      dev::k_sieve<false><<<(u32)b.nt, cfg.block, b.lds, st>>>(b.d_tiles, b.d_hdr, m.d_lays, b.d_tile_counts,
+                                                               b.d_surv_p, b.d_surv_w, (u32)b.surv_cap,
+                                                               b.d_scalars, b.d_scalars + 1, b.lds_words,
+                                                               b.off_words);
+    }
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(b.ev2, st));
+    dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
+                                                           b.d_one);
+    HIP_TRY(hipGetLastError());
+    if (cfg.device_tail) {
+      // k_rung finishes the Miller-Rabin bases of the pending values itself.
+      dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, st>>>(
+          b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags, b.d_one, m.trial, b.d_rung, b.d_scalars + 2);
+    } else {
+      dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
+                                                             b.d_one);
+    }
+    HIP_TRY(hipGetLastError());
+    dev::k_pack<<<std::max(1, cus_), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
+                                                   b.d_surv_p, b.d_surv_w, b.d_flags, b.d_rung,
+                                                   cfg.device_tail ? 1u : 0u, b.dh_count, b.dh_tile_counts,
+                                                   b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  };
+  HIP_TRY(hipEventRecord(b.ev0, m.stream));
+  if (cfg.graph && cfg.persistent) {
+    if (!b.graph_ok || b.graph_block != cfg.block || b.graph_lds != b.lds || b.graph_tail != cfg.device_tail ||
+        b.graph_grid != b.grid) {
+      if (b.graph_exec) { HIP_TRY(hipGraphExecDestroy(b.graph_exec)); b.graph_exec = nullptr; }
+      if (b.graph) { HIP_TRY(hipGraphDestroy(b.graph)); b.graph = nullptr; }
+      HIP_TRY(hipStreamBeginCapture(m.capture, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue(m.capture, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(m.capture, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) { err_ = std::string("hipStreamEndCapture: ") + hipGetErrorString(e); return -1; }
+      b.graph = g;
+      HIP_TRY(hipGraphInstantiate(&b.graph_exec, b.graph, nullptr, nullptr, 0));
+      b.graph_ok = true;
+      b.graph_block = cfg.block;
+      b.graph_lds = b.lds;
+      b.graph_tail = cfg.device_tail;
+      b.graph_grid = b.grid;
+    }
+    HIP_TRY(hipGraphLaunch(b.graph_exec, m.stream));
+  } else if (enqueue(m.stream, true)) {
+    return -1;
+  }
+  HIP_TRY(hipEventRecord(b.ev3, m.stream));
+  return 0;
+}
+
+int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
+  if (m.busy == 2) { err_ = "two batches are already in flight"; return -1; }
   HIP_TRY(hipSetDevice(device_));
   const size_t nt = tiles.size();
-  out.counts.assign(nwin, WinCounts{0, 0, 0});
-  out.surv_p.clear();
-  out.surv_w.clear();
-  out.surv_prime.clear();
-  out.rung.clear();
-  out.replays = 0;
-  out.device_ms = out.sieve_ms = 0;
   if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
       cfg.max_bits > kMaxTileBits) {
     err_ = "bad launch configuration";
@@ -1351,180 +1485,132 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
       return -1;
     }
   }
-  if (nt == 0) return 0;
   if (nt > 0xFFFFFFFFull || nwin > 0x55555555u) { err_ = "batch too large"; return -1; }
-  // Capacities. Any reallocation invalidates a captured graph.
-  // Pinned host buffer with its device-side address (for k_pack's stores).
-  auto pinned = [&](void **h, void **d, size_t bytes) -> int {
-    if (*h) HIP_TRY(hipHostFree(*h));
-    *h = nullptr;
-    HIP_TRY(hipHostMalloc(h, bytes, hipHostMallocMapped));
-    if (d) HIP_TRY(hipHostGetDevicePointer(d, *h, 0));
-    return 0;
-  };
-  if (nt > m.tiles_cap) {
-    if (m.d_tiles) HIP_TRY(hipFree(m.d_tiles));
-    m.tiles_cap = std::max<size_t>(nt, 2 * m.tiles_cap);
-    HIP_TRY(hipMalloc(&m.d_tiles, m.tiles_cap * sizeof(TileDesc)));
-    if (m.d_tile_counts) HIP_TRY(hipFree(m.d_tile_counts));
-    HIP_TRY(hipMalloc(&m.d_tile_counts, m.tiles_cap * sizeof(uint4)));
-    if (pinned((void **)&m.h_tiles, nullptr, m.tiles_cap * sizeof(TileDesc)) ||
-        pinned((void **)&m.h_tile_counts, (void **)&m.dh_tile_counts, m.tiles_cap * sizeof(uint4)))
-      return -1;
-    m.graph_ok = false;
-  }
-  auto grow_surv = [&](size_t need) -> int {
-    if (need <= m.surv_cap) return 0;
-    size_t cap = std::max<size_t>(m.surv_cap ? m.surv_cap : (1u << 16), 1u << 16);
-    while (cap < need) cap *= 2;
-    if (cap > (1ull << 28)) { err_ = "survivor count exceeds 2^28 per batch"; return -1; }
-    for (void *p : {(void *)m.d_surv_p, (void *)m.d_surv_w, (void *)m.d_flags, (void *)m.d_one, (void *)m.d_rung})
-      if (p) HIP_TRY(hipFree(p));
-    HIP_TRY(hipMalloc(&m.d_surv_p, cap * sizeof(u64)));
-    HIP_TRY(hipMalloc(&m.d_one, cap * sizeof(u64)));
-    HIP_TRY(hipMalloc(&m.d_rung, cap * sizeof(dev::RungOut)));
-    HIP_TRY(hipMalloc(&m.d_surv_w, cap * sizeof(u32)));
-    HIP_TRY(hipMalloc(&m.d_flags, cap));
-    if (pinned((void **)&m.h_surv_p, (void **)&m.dh_surv_p, cap * sizeof(u64)) ||
-        pinned((void **)&m.h_surv_w, (void **)&m.dh_surv_w, cap * sizeof(u32)) ||
-        pinned((void **)&m.h_flags, (void **)&m.dh_flags, cap) ||
-        pinned((void **)&m.h_rung, (void **)&m.dh_rung, cap * sizeof(dev::RungOut)))
-      return -1;
-    m.surv_cap = cap;
-    m.graph_ok = false;
-    return 0;
-  };
-  if (grow_surv(1u << 16)) return -1;
+  const int si = (m.head + m.busy) & 1;
+  Slot &b = m.slot[si];
+  b.nt = nt;
+  b.nwin = nwin;
+  b.cfg = cfg;
+  if (nt > 0) {
+    // Capacities. Any reallocation invalidates the slot's captured graph.
+    if (nt > b.tiles_cap) {
+      if (b.d_tiles) HIP_TRY(hipFree(b.d_tiles));
+      b.tiles_cap = std::max<size_t>(nt, 2 * b.tiles_cap);
+      HIP_TRY(hipMalloc(&b.d_tiles, b.tiles_cap * sizeof(TileDesc)));
+      if (b.d_tile_counts) HIP_TRY(hipFree(b.d_tile_counts));
+      HIP_TRY(hipMalloc(&b.d_tile_counts, b.tiles_cap * sizeof(uint4)));
+      HIP_TRY(pinned_alloc((void **)&b.h_tiles, nullptr, b.tiles_cap * sizeof(TileDesc)));
+      HIP_TRY(pinned_alloc((void **)&b.h_tile_counts, (void **)&b.dh_tile_counts, b.tiles_cap * sizeof(uint4)));
+      b.graph_ok = false;
+    }
+    if (grow_surv(si, 1u << 16)) return -1;
+    // The slot's previous batch has been finished, so its staging buffers are free.
+    b.h_hdr->ntiles = (u32)nt;
+    b.h_hdr->nwin = nwin;
+    b.h_hdr->has_sieve = has_sieve ? 1 : 0;
+    b.h_hdr->pad = 0;
+    HIP_TRY(hipMemcpyAsync(b.d_hdr, b.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
+    std::memcpy(b.h_tiles, tiles.data(), nt * sizeof(TileDesc));
+    HIP_TRY(hipMemcpyAsync(b.d_tiles, b.h_tiles, nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
 
-  m.h_hdr->ntiles = (u32)nt;
-  m.h_hdr->nwin = nwin;
-  m.h_hdr->has_sieve = has_sieve ? 1 : 0;
-  m.h_hdr->pad = 0;
-  HIP_TRY(hipMemcpyAsync(m.d_hdr, m.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
-  std::memcpy(m.h_tiles, tiles.data(), nt * sizeof(TileDesc));
-  HIP_TRY(hipMemcpyAsync(m.d_tiles, m.h_tiles, nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
-
-  // One LDS byte per position, sized to the batch's largest tile (rounded to whole
-  // 32-position chunks): short tiles then leave room for more resident workgroups.
-  u32 tile_max = 0;
-  for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
-  const u32 lds_bytes = (tile_max + 31u) & ~31u;
-  const u32 lds_words = lds_bytes / 4;
-  const u32 off_words = m.off_max / 2;
-  const size_t lds = lds_bytes + off_words * 4 + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
-  if (lds > lds_limit_) {
-    err_ = "tile of " + std::to_string(tile_max) + " positions and " + std::to_string(m.off_max) +
-           " filter offsets need " + std::to_string(lds) + " B of LDS, above the device's " +
-           std::to_string(lds_limit_);
-    return -1;
-  }
-  if (m.occ_block != cfg.block || m.occ_bits != lds) {
-    if (lds > 65536) {
-      HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // One LDS byte per position, sized to the batch's largest tile (rounded to whole
+    // 32-position chunks): short tiles then leave room for more resident workgroups.
+    u32 tile_max = 0;
+    for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
+    const u32 lds_bytes = (tile_max + 31u) & ~31u;
+    b.lds_words = lds_bytes / 4;
+    b.off_words = m.off_max / 2;
+    b.lds = lds_bytes + b.off_words * 4 + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
+    if (b.lds > lds_limit_) {
+      err_ = "tile of " + std::to_string(tile_max) + " positions and " + std::to_string(m.off_max) +
+             " filter offsets need " + std::to_string(b.lds) + " B of LDS, above the device's " +
+             std::to_string(lds_limit_);
+      return -1;
     }
-    int nb = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, lds));
-    m.occupancy_blocks = std::max(1, nb);
-    m.occ_block = cfg.block;
-    m.occ_bits = (u32)lds;
-    m.graph_ok = false;
-  }
-  const u32 pgrid = (u32)((size_t)cus_ * m.occupancy_blocks);
-  // The device part of one batch: reset, sieve, Miller-Rabin. The persistent grid does
-  // not depend on the batch, which is what lets a graph of it be replayed.
-  auto enqueue = [&](bool timed) -> int {
-    dev::k_reset<<<1, 64, 0, m.stream>>>(m.d_scalars);
-    HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(m.ev1, m.stream));
-    if (cfg.persistent) {
-      dev::k_sieve<true><<<pgrid, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_tile_counts,
-                                                              m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                              m.d_scalars, m.d_scalars + 1, lds_words, off_words);
-    } else {
-      dev::k_sieve<false><<<(u32)nt, cfg.block, lds, m.stream>>>(m.d_tiles, m.d_hdr, m.d_lays, m.d_tile_counts,
-                                                                 m.d_surv_p, m.d_surv_w, (u32)m.surv_cap,
-                                                                 m.d_scalars, m.d_scalars + 1, lds_words, off_words);
-    }
-    HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(m.ev2, m.stream));
-    dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
-                                                                  m.d_flags, m.d_one);
-    HIP_TRY(hipGetLastError());
-    if (cfg.device_tail) {
-      // k_rung finishes the Miller-Rabin bases of the pending values itself.
-      dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, m.stream>>>(
-          m.d_surv_p, m.d_scalars, (u32)m.surv_cap, m.d_flags, m.d_one, m.trial, m.d_rung, m.d_scalars + 2);
-    } else {
-      dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, m.stream>>>(m.d_surv_p, m.d_scalars, (u32)m.surv_cap,
-                                                                    m.d_flags, m.d_one);
-    }
-    HIP_TRY(hipGetLastError());
-    dev::k_pack<<<std::max(1, cus_), 256, 0, m.stream>>>(m.d_hdr, m.d_tile_counts, m.d_scalars, (u32)m.surv_cap,
-                                                         m.d_surv_p, m.d_surv_w, m.d_flags, m.d_rung,
-                                                         cfg.device_tail ? 1u : 0u, m.dh_count, m.dh_tile_counts,
-                                                         m.dh_surv_p, m.dh_surv_w, m.dh_flags, m.dh_rung);
-    HIP_TRY(hipGetLastError());
-    return 0;
-  };
-  const bool use_graph = cfg.graph && cfg.persistent;
-  for (;;) {
-    HIP_TRY(hipEventRecord(m.ev0, m.stream));
-    if (use_graph) {
-      if (!m.graph_ok || m.graph_block != cfg.block || m.graph_bits != lds || m.graph_tail != cfg.device_tail) {
-        if (m.graph_exec) { HIP_TRY(hipGraphExecDestroy(m.graph_exec)); m.graph_exec = nullptr; }
-        if (m.graph) { HIP_TRY(hipGraphDestroy(m.graph)); m.graph = nullptr; }
-        HIP_TRY(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue(false);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(m.stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) { err_ = std::string("hipStreamEndCapture: ") + hipGetErrorString(e); return -1; }
-        m.graph = g;
-        HIP_TRY(hipGraphInstantiate(&m.graph_exec, m.graph, nullptr, nullptr, 0));
-        m.graph_ok = true;
-        m.graph_block = cfg.block;
-        m.graph_bits = (u32)lds;
-        m.graph_tail = cfg.device_tail;
+    if (m.occ_block != cfg.block || m.occ_lds != b.lds) {
+      if (b.lds > 65536) {
+        HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)b.lds));
+        HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)b.lds));
       }
-      HIP_TRY(hipGraphLaunch(m.graph_exec, m.stream));
-    } else {
-      if (enqueue(true)) return -1;
+      int nb = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, b.lds));
+      m.occupancy_blocks = std::max(1, nb);
+      m.occ_block = cfg.block;
+      m.occ_lds = b.lds;
     }
-    HIP_TRY(hipEventRecord(m.ev3, m.stream));
-    HIP_TRY(hipStreamSynchronize(m.stream));
+    b.grid = (u32)((size_t)cus_ * m.occupancy_blocks);
+    if (enqueue_batch(si)) return -1;
+  }
+  ++m.busy;
+  return 0;
+}
+
+int DeviceCtx::finish(BatchResult &out) {
+  Impl &m = *impl_;
+  if (device_ < 0) { err_ = "device not open"; return -1; }
+  if (m.busy == 0) { err_ = "no batch in flight"; return -1; }
+  HIP_TRY(hipSetDevice(device_));
+  const int si = m.head;
+  Slot &b = m.slot[si];
+  // The slot is released whatever happens below. With nothing left in flight the next
+  // batch takes slot 0 again, so one-at-a-time use keeps to one slot and one graph.
+  m.head ^= 1;
+  if (--m.busy == 0) m.head = 0;
+  out.counts.assign(b.nwin, WinCounts{0, 0, 0});
+  out.surv_p.clear();
+  out.surv_w.clear();
+  out.surv_prime.clear();
+  out.rung.clear();
+  out.replays = 0;
+  out.device_ms = out.sieve_ms = 0;
+  if (b.nt == 0) return 0;
+  const bool use_graph = b.cfg.graph && b.cfg.persistent;
+  for (;;) {
+    // Waits for this batch only: a later batch keeps running meanwhile.
+    HIP_TRY(hipEventSynchronize(b.ev3));
     float all = 0, sv = 0;
-    HIP_TRY(hipEventElapsedTime(&all, m.ev0, m.ev3));
-    if (!use_graph) HIP_TRY(hipEventElapsedTime(&sv, m.ev1, m.ev2));
+    HIP_TRY(hipEventElapsedTime(&all, b.ev0, b.ev3));
+    if (!use_graph) HIP_TRY(hipEventElapsedTime(&sv, b.ev1, b.ev2));
     out.device_ms += all;
     out.sieve_ms += use_graph ? -1.0 : sv;
-    const u32 cnt = *m.h_count;
-    if (cnt <= m.surv_cap) break;
+    const u32 cnt = *b.h_count;
+    if (cnt <= b.surv_cap) break;
     // Overflow: the counts are complete, the list is not. Grow and replay the batch
     // (sieve/cuda/scan.cu:223-242 does the same).
     ++out.replays;
-    if (grow_surv(cnt)) return -1;
+    if (grow_surv(si, cnt) || enqueue_batch(si)) return -1;
   }
-  // Everything is in the pinned mirrors now (k_pack ran before the synchronisation).
-  const u32 cnt = *m.h_count;
-  out.surv_p.assign(m.h_surv_p, m.h_surv_p + cnt);
-  out.surv_w.assign(m.h_surv_w, m.h_surv_w + cnt);
+  // Everything is in the pinned mirrors now (k_pack ran before ev3).
+  const u32 cnt = *b.h_count;
+  out.surv_p.assign(b.h_surv_p, b.h_surv_p + cnt);
+  out.surv_w.assign(b.h_surv_w, b.h_surv_w + cnt);
   out.surv_prime.resize(cnt);
-  for (u32 i = 0; i < cnt; ++i) out.surv_prime[i] = m.h_flags[i] != 0;   // pending after every base == prime
-  if (cfg.device_tail) {
+  for (u32 i = 0; i < cnt; ++i) out.surv_prime[i] = b.h_flags[i] != 0;   // pending after every base == prime
+  if (b.cfg.device_tail) {
     static_assert(sizeof(dev::RungOut) == sizeof(RungResult), "RungOut mirrors RungResult");
     out.rung.resize(cnt);
     for (u32 i = 0; i < cnt; ++i)
-      if (out.surv_prime[i]) std::memcpy(&out.rung[i], &m.h_rung[i], sizeof(RungResult));
+      if (out.surv_prime[i]) std::memcpy(&out.rung[i], &b.h_rung[i], sizeof(RungResult));
   }
-  for (size_t i = 0; i < nt; ++i) {
-    WinCounts &c = out.counts[tiles[i].win];
-    c.positions += m.h_tile_counts[i].x;
-    c.after_covers += m.h_tile_counts[i].y;
-    c.survivors += m.h_tile_counts[i].z;
+  for (size_t i = 0; i < b.nt; ++i) {
+    WinCounts &c = out.counts[b.h_tiles[i].win];
+    c.positions += b.h_tile_counts[i].x;
+    c.after_covers += b.h_tile_counts[i].y;
+    c.survivors += b.h_tile_counts[i].z;
   }
   return 0;
 }
+
+int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
+                   BatchResult &out) {
+  if (impl_->busy) { err_ = "batches are in flight: finish them before run"; return -1; }
+  if (launch(tiles, nwin, has_sieve, cfg)) return -1;
+  return finish(out);
+}
+
+int DeviceCtx::in_flight() const { return impl_->busy; }
 
 int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   Impl &m = *impl_;
@@ -1564,8 +1650,9 @@ int DeviceCtx::mem_info(size_t &free_b, size_t &total_b) {
 size_t DeviceCtx::resident_bytes() const {
   const Impl &m = *impl_;
   const size_t per_surv = sizeof(u64) * 2 + sizeof(u32) + 1 + sizeof(dev::RungOut);
-  return layout_bytes_ + m.trial.n * (sizeof(uint2) + 16) + m.tiles_cap * sizeof(TileDesc) +
-         m.tiles_cap * sizeof(uint4) + m.surv_cap * per_surv;
+  size_t batch = 0;
+  for (const Slot &b : m.slot) batch += b.tiles_cap * (sizeof(TileDesc) + sizeof(uint4)) + b.surv_cap * per_surv;
+  return layout_bytes_ + m.trial.n * (sizeof(uint2) + 16) + batch;
 }
 
 int DeviceCtx::lds_scatter_peak(double &stores_per_s) {
@@ -1577,13 +1664,13 @@ int DeviceCtx::lds_scatter_peak(double &stores_per_s) {
   HIP_TRY(hipMalloc(&sink, blocks * sizeof(u32)));
   dev::k_lds_scatter<<<blocks, 512, 0, m.stream>>>(64, sink);   // warm-up
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m.ev0, m.stream));
+  HIP_TRY(hipEventRecord(m.pev0, m.stream));
   dev::k_lds_scatter<<<blocks, 512, 0, m.stream>>>(iters, sink);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m.ev1, m.stream));
+  HIP_TRY(hipEventRecord(m.pev1, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, m.ev0, m.ev1));
+  HIP_TRY(hipEventElapsedTime(&ms, m.pev0, m.pev1));
   HIP_TRY(hipFree(sink));
   stores_per_s = (double)blocks * 512.0 * iters / (ms * 1e-3);
   return 0;
@@ -1598,13 +1685,13 @@ int DeviceCtx::lds_peak(double &ops_per_s) {
   HIP_TRY(hipMalloc(&sink, blocks * sizeof(u32)));
   dev::k_lds_peak<<<blocks, 256, 0, m.stream>>>(64, sink);   // warm-up
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m.ev0, m.stream));
+  HIP_TRY(hipEventRecord(m.pev0, m.stream));
   dev::k_lds_peak<<<blocks, 256, 0, m.stream>>>(iters, sink);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m.ev1, m.stream));
+  HIP_TRY(hipEventRecord(m.pev1, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, m.ev0, m.ev1));
+  HIP_TRY(hipEventElapsedTime(&ms, m.pev0, m.pev1));
   HIP_TRY(hipFree(sink));
   ops_per_s = (double)blocks * 256.0 * iters / (ms * 1e-3);
   return 0;

@@ -326,11 +326,15 @@ int Scanner::ensure_layout(u64 Mp, std::string &msg) {
   return id;
 }
 
-int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
-                         ScanTiming &tm, std::string &msg) {
+int Scanner::device_submit(const std::vector<Interval> &wins, std::string &msg) {
   const double t0 = now_ms();
-  tm = ScanTiming();
+  pending_.emplace_back();
+  Pending &pb = pending_.back();
+  pb.wins = wins;
+  ScanTiming &tm = pb.tm;
+  std::vector<WindowRecord> &out = pb.out;
   out.assign(wins.size(), WindowRecord());
+  auto fail = [&]() { pending_.pop_back(); return -1; };
   const u32 maxb = tune_.launch.max_bits;
   // Tiles go out longest first, so the persistent queue ends with the short ones. Only
   // load balance depends on that order (never a result), so tiles are placed straight
@@ -348,7 +352,7 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
     if (r.hi <= r.lo) continue;
     const u64 Mp = choose_layout(r.hi - r.lo);
     const int id = ensure_layout(Mp, msg);
-    if (id < 0) return -1;
+    if (id < 0) return fail();
     ids[w] = id;
     r.layout_Mp = Mp;
     const Layout &L = *layouts_[Mp];
@@ -378,13 +382,31 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
   const double t1 = now_ms();
   tm.plan_ms = t1 - t0;
 
-  if (!dev_open_) { msg = "FATAL: HIP device not open"; return -1; }
-  if (dev_.run(tiles, (u32)wins.size(), plan_.filters.mod.size() != plan_.filters.ncov, tune_.launch, br)) {
+  if (!dev_open_) { msg = "FATAL: HIP device not open"; return fail(); }
+  if (dev_.launch(tiles, (u32)wins.size(), plan_.filters.mod.size() != plan_.filters.ncov, tune_.launch)) {
+    msg = "FATAL: HIP device: " + dev_.last_error();
+    return fail();
+  }
+  tm.batch_ms = now_ms() - t1;
+  tm.total_ms = now_ms() - t0;
+  return 0;
+}
+
+int Scanner::device_collect(std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                            ScanTiming &tm, std::string &msg) {
+  if (pending_.empty()) { msg = "FATAL: no batch in flight"; return -1; }
+  Pending pb = std::move(pending_.front());
+  pending_.pop_front();
+  const double t1 = now_ms();
+  wins = std::move(pb.wins);
+  out = std::move(pb.out);
+  tm = pb.tm;
+  if (dev_.finish(br)) {
     msg = "FATAL: HIP device: " + dev_.last_error();
     return -1;
   }
   const double t2 = now_ms();
-  tm.batch_ms = t2 - t1;
+  tm.batch_ms += t2 - t1;
   tm.device_ms = br.device_ms;
   tm.sieve_ms = br.sieve_ms;
   tm.replays = br.replays;
@@ -406,8 +428,16 @@ int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRe
   }
   for (size_t i = 0; i < br.surv_p.size(); ++i)
     if (!br.surv_prime[i]) ++out[br.surv_w[i]].composite;
-  tm.total_ms = now_ms() - t0;
+  tm.total_ms += now_ms() - t1;
   return 0;
+}
+
+int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                         ScanTiming &tm, std::string &msg) {
+  if (!pending_.empty()) { msg = "FATAL: batches are in flight: collect them first"; return -1; }
+  std::vector<Interval> w;
+  if (device_submit(wins, msg)) return -1;
+  return device_collect(w, out, br, tm, msg);
 }
 
 int Scanner::survivors(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, std::vector<u64> &p,
@@ -432,8 +462,26 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
                   std::string &msg) {
   BatchResult br;
   if (device_pass(wins, out, br, tm, msg)) return -1;
+  finish_records(wins, out, br, tm);
+  return 0;
+}
+
+int Scanner::submit(const std::vector<Interval> &wins, std::string &msg) {
+  if (pending_.size() >= 2) { msg = "FATAL: two batches are already in flight"; return -1; }
+  return device_submit(wins, msg);
+}
+
+int Scanner::collect(std::vector<WindowRecord> &out, ScanTiming &tm, std::string &msg) {
+  BatchResult br;
+  std::vector<Interval> wins;
+  if (device_collect(wins, out, br, tm, msg)) return -1;
+  finish_records(wins, out, br, tm);
+  return 0;
+}
+
+void Scanner::finish_records(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                             ScanTiming &tm) {
   const double t2 = now_ms();
-  const double t0 = t2 - tm.total_ms;
   std::vector<size_t> prime_idx;
   for (size_t i = 0; i < br.surv_p.size(); ++i)
     if (br.surv_prime[i]) prime_idx.push_back(i);
@@ -490,8 +538,7 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
       r.line_p.push_back(k.p);
     }
   }
-  tm.total_ms = now_ms() - t0;
-  return 0;
+  tm.total_ms += now_ms() - t2;
 }
 
 void Scanner::emit_key(u64 p, u64 lo, u64 &k1, u64 &k2) const {

@@ -2,6 +2,7 @@
 // layout per window, cut windows into tiles, run the device batch, certify the primes
 // on the host, and assemble each window's record exactly as rung_scan3 prints it.
 #pragma once
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -61,6 +62,14 @@ class Scanner {
   // -1 on a device error (msg).
   int scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, ScanTiming &tm,
            std::string &msg);
+  // Pipelined scan: submit plans a batch and launches it without waiting (at most two
+  // batches in flight); collect finishes the oldest submitted batch into the records
+  // scan would have produced. The host tail of one batch then runs while the device
+  // works on the next (sieve/cuda/scan.cu:1-6, 197-207). scan refuses while a batch is
+  // in flight.
+  int submit(const std::vector<Interval> &wins, std::string &msg);
+  int collect(std::vector<WindowRecord> &out, ScanTiming &tm, std::string &msg);
+  size_t in_flight() const { return pending_.size(); }
   // Device stages only: each window's counters (positions..composite) and all stage-C
   // survivors of the batch, ascending by (window, p), with their primality.
   int survivors(const std::vector<Interval> &wins, std::vector<WindowRecord> &out,
@@ -106,6 +115,19 @@ class Scanner {
   // Tiles + device batch + counters; composite counts filled, primes left in br.
   int device_pass(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
                   ScanTiming &tm, std::string &msg);
+  // device_pass in two halves: plan and launch, then finish and count.
+  int device_submit(const std::vector<Interval> &wins, std::string &msg);
+  int device_collect(std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                     ScanTiming &tm, std::string &msg);
+  // Stage D leftovers on the host, then the emission order, into out's lines.
+  void finish_records(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
+                      ScanTiming &tm);
+  struct Pending {
+    std::vector<Interval> wins;
+    std::vector<WindowRecord> out;   // lo, hi and layout filled at submission
+    ScanTiming tm;                   // plan and launch times so far
+  };
+  std::deque<Pending> pending_;
   Plan plan_;
   Tuning tune_;
   DeviceCtx dev_;

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -403,11 +404,25 @@ int main(int argc, char **argv) {
       if (o.emit_surv || l.text.compare(0, 9, "SURVIVOR ") != 0) fwrite(l.text.data(), 1, l.text.size(), stdout);
     all.clear();
   };
-  for (u64 a = o.lo; a < o.hi;) {
-    const u64 b = o.hi - a > o.chunk ? a + o.chunk : o.hi;
+  // Chunk k+1 is submitted before chunk k is collected, so the host tail of one chunk
+  // overlaps the device work of the next (sieve/cuda/scan.cu:1-6).
+  std::deque<Interval> inflight;
+  u64 next = o.lo;
+  auto submit_next = [&]() -> bool {
+    const u64 b = o.hi - next > o.chunk ? next + o.chunk : o.hi;
+    if (sc.submit({Interval{next, b}}, msg)) return false;
+    inflight.push_back(Interval{next, b});
+    next = b;
+    return true;
+  };
+  if (next < o.hi && !submit_next()) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
+  while (!inflight.empty()) {
+    if (next < o.hi && !submit_next()) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
+    const u64 a = inflight.front().lo, b = inflight.front().hi;
+    inflight.pop_front();
     std::vector<WindowRecord> recs;
     ScanTiming tm;
-    if (sc.scan({Interval{a, b}}, recs, tm, msg)) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
+    if (sc.collect(recs, tm, msg)) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
     const WindowRecord &r = recs[0];
     total.positions += r.positions; total.covered += r.covered; total.sieved += r.sieved; total.mr += r.mr;
     total.composite += r.composite; total.rung += r.rung; total.direct += r.direct;
@@ -423,7 +438,6 @@ int main(int argc, char **argv) {
     if (o.profile)
       fprintf(stderr, "[es_sieve] chunk [%" PRIu64 ",%" PRIu64 ") layout=%" PRIu64 " tiles=%" PRIu64
               " survivors=%" PRIu64 " primes=%" PRIu64 "\n", a, b, r.layout_Mp, tm.tiles, tm.survivors, tm.primes);
-    a = b;
     if (stream) flush_lines();
   }
   flush_lines();

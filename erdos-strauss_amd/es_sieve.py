@@ -90,6 +90,10 @@ def lib():
             "es_scan_records": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, ctypes.c_int,
                                                P(ctypes.c_void_p), P(ctypes.c_size_t), P(ctypes.c_size_t),
                                                P(ctypes.c_int)]),
+            "es_scan_submit": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t]),
+            "es_scan_collect": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_void_p), P(ctypes.c_size_t),
+                                               P(ctypes.c_size_t), P(ctypes.c_int), P(ctypes.c_size_t)]),
+            "es_scan_in_flight": (ctypes.c_int, [ctypes.c_void_p]),
             "es_scan_counts": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, P(EsCounts)]),
             "es_survivors": (ctypes.c_int, [ctypes.c_void_p, U64, U64, P(U64), P(ctypes.c_uint8), ctypes.c_size_t,
                                             P(ctypes.c_size_t), P(EsCounts)]),
@@ -163,6 +167,7 @@ class Scanner:
         _check(L.es_scanner_open(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self.wheel = wheel
+        self._sizes = []          # window counts of the batches submitted and not collected
 
     def close(self):
         if self.h:
@@ -204,6 +209,41 @@ class Scanner:
             out.append(raw[pos:pos + rl[i]].decode())
             pos += rl[i]
         return out, [ec[i] for i in range(n)]
+
+    def submit(self, windows):
+        """Pipelined records(): plans [(lo, hi)] and launches the batch without waiting.
+        At most two batches are in flight; collect() returns them in submission order."""
+        n = len(windows)
+        _check(lib().es_scan_submit(self.h, _u64s([w[0] for w in windows]), _u64s([w[1] for w in windows]), n))
+        self._sizes.append(n)
+
+    def collect(self, emit_survivors=True):
+        """The oldest submitted batch's (records, exit codes), exactly as records() returns them."""
+        if not self._sizes:
+            raise EsError("no batch in flight")
+        n = self._sizes[0]
+        text = ctypes.c_void_p()
+        tl = ctypes.c_size_t()
+        rl = (ctypes.c_size_t * max(1, n))()
+        ec = (ctypes.c_int * max(1, n))()
+        got = ctypes.c_size_t()
+        try:
+            _check(lib().es_scan_collect(self.h, 1 if emit_survivors else 0, ctypes.byref(text), ctypes.byref(tl),
+                                         rl, ec, ctypes.byref(got)))
+        finally:
+            self._sizes.pop(0)
+        try:
+            raw = ctypes.string_at(text, tl.value)
+        finally:
+            lib().es_free(text)
+        out, pos = [], 0
+        for i in range(got.value):
+            out.append(raw[pos:pos + rl[i]].decode())
+            pos += rl[i]
+        return out, [ec[i] for i in range(got.value)]
+
+    def in_flight(self):
+        return lib().es_scan_in_flight(self.h)
 
     def counts(self, windows):
         n = len(windows)

@@ -202,6 +202,69 @@ def test_graph_replays_match_eager():
         sc.set_graph(False)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_pipelined_batches_match_records(graph):
+    """submit/collect with two batches in flight (the host tail of one batch overlapping
+    the device work of the next) returns, batch by batch and in submission order, the
+    records records() gives one batch at a time: uneven batches, a batch with an empty
+    window, an empty batch, each slot reused several times, graph on and off."""
+    kw = dict(wheel=2042040, mmax=10000, sieve=0, filters=F204)
+    sc = scanner(**kw)
+    batches = [[(10 ** 15, 10 ** 15 + 2 ** 30)], [(10 ** 13, 10 ** 13 + 10 ** 9), (10 ** 16, 10 ** 16)],
+               [(TOP - 2 ** 28, TOP)], [], [(10 ** 16, 10 ** 16 + 2 ** 24), (10 ** 12, 10 ** 12 + 2 ** 20)],
+               [(10 ** 15, 10 ** 15 + 2 ** 30)]]
+    want = [sc.records(b) for b in batches]
+    sc.set_graph(graph)
+    try:
+        for rep in range(2):
+            print("pipelined", graph, rep, flush=True)
+            sc.submit(batches[0])
+            for i in range(len(batches)):
+                if i + 1 < len(batches):
+                    sc.submit(batches[i + 1])
+                assert sc.in_flight() == (2 if i + 1 < len(batches) else 1)
+                assert sc.collect() == want[i], (rep, i)
+            assert sc.in_flight() == 0
+    finally:
+        sc.set_graph(False)
+
+
+def test_pipelined_refusals():
+    """At most two batches in flight; no collect without a submission; the one-shot entry
+    points refuse while a batch is in flight and work again once it is collected."""
+    sc = scanner(wheel=2042040, mmax=10000, sieve=0, filters=F204)
+    w = [(10 ** 13, 10 ** 13 + 2 ** 24)]
+    want = sc.records(w)
+    with pytest.raises(es.EsError):
+        sc.collect()
+    sc.submit(w)
+    sc.submit(w)
+    with pytest.raises(es.EsError):
+        sc.submit(w)
+    with pytest.raises(es.EsError):
+        sc.records(w)
+    assert sc.collect() == want
+    assert sc.collect() == want
+    assert sc.records(w) == want
+
+
+def test_pipelined_survivor_overflow_replay():
+    """A batch that overflows its fresh survivor list while the next batch is already in
+    flight replays on its own slot; both batches come back as records() gives them."""
+    big = [(10 ** 9, 10 ** 9 + 4 * 10 ** 8)]
+    small = [(10 ** 12, 10 ** 12 + 2 ** 24)]
+    kw = dict(wheel=120120, mmax=60, sieve=0, filters=None)
+    want_big = oracle_record(big[0][0], big[0][1], kw)
+    with es.Scanner(device=0, **kw) as sc:
+        sc.submit(big)
+        sc.submit(small)
+        (rec,), (code,) = sc.collect()
+        assert sc.timing()["replays"] >= 1 and sc.timing()["survivors"] > 65536
+        assert (rec, code) == want_big
+        got_small = sc.collect()
+        assert got_small == sc.records(small)
+
+
 def test_survivor_overflow_replay():
     """More survivors than the initial device list holds: counts are complete, the
     batch replays with a larger list (scan.cu:223-242), output unchanged."""
@@ -314,6 +377,17 @@ def test_cli_round_order_across_chunks():
     rc, out, err = paths.run(paths.SIEVE_BIN, argv, timeout=300)
     assert rc == 0, err.decode()[-2000:]
     assert out.decode() == open(W19).read()
+
+
+def test_cli_lane_order_across_chunks():
+    """Lane order holds every chunk's lines to the end: the chunked CLI run (eight
+    chunks, pipelined two deep) gives the record of the whole window in one batch."""
+    with es.Scanner(wheel=2042040, filters=F204, device=0) as sc:
+        (want,), (ec,) = sc.records([(W19_LO, W19_HI)])
+    argv = [str(W19_LO), str(W19_HI), "--filters", os.path.relpath(F204, paths.REPO), "--chunk", str(1 << 37)]
+    rc, out, err = paths.run(paths.SIEVE_BIN, argv, timeout=300)
+    assert rc == ec, err.decode()[-2000:]
+    assert out.decode() == want
 
 
 def test_top_of_range_window_reproduces_published_digests():
