@@ -283,11 +283,21 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   }
 }
 
-// Zeroes the survivor count and the tile queue (a fill kernel rather than a memset, so
-// a captured graph holds no memset node). Every other per-batch output is written in
-// full by the kernels, so one captured graph serves every batch that fits the buffers.
-__global__ void k_reset(u32 *scalars) {
-  if (blockIdx.x == 0 && threadIdx.x < 4) scalars[threadIdx.x] = 0;   // counts and queue heads
+// Stages one batch: zeroes the survivor count and the queue heads, and copies the batch
+// header and the tile descriptors from pinned host memory (written by the host before
+// the launch) into device memory. A kernel rather than a memset and two copies, so a
+// captured graph holds no memset or copy node and no copy engine has to be waited for
+// between batches. The tile count comes from the header itself, so one captured graph
+// serves every batch that fits the buffers; the host keeps ntiles within the buffers'
+// capacity. Every other per-batch output is written in full by the kernels.
+__global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
+                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars) {
+  const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  const BatchHdr h = *h_hdr;
+  if (g == 0) *hdr = h;
+  if (g < 4) scalars[g] = 0;   // survivor count, tile queue, stage-D queue
+  const u32 n = h.ntiles * (u32)(sizeof(TileDesc) / sizeof(uint2));
+  for (u32 i = g; i < n; i += stride) tiles[i] = h_tiles[i];
 }
 
 // ------------------------------------------------------------------ Miller-Rabin
@@ -1173,6 +1183,8 @@ struct Slot {
   // writes the results into (device-side addresses in the dh_* twins).
   u32 *h_count = nullptr;
   TileDesc *h_tiles = nullptr;
+  TileDesc *dh_tiles = nullptr;
+  dev::BatchHdr *dh_hdr = nullptr;
   uint4 *h_tile_counts = nullptr;
   u64 *h_surv_p = nullptr;
   u32 *h_surv_w = nullptr;
@@ -1267,7 +1279,8 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipEventCreate(&b.ev2));
     HIP_TRY(hipEventCreate(&b.ev3));
     HIP_TRY(hipMalloc(&b.d_hdr, sizeof(dev::BatchHdr)));
-    HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr)));
+    HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
+    HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
     HIP_TRY(hipMalloc(&b.d_scalars, 4 * sizeof(u32)));
     HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
@@ -1394,7 +1407,7 @@ int DeviceCtx::grow_surv(int si, size_t need) {
   return 0;
 }
 
-// Enqueues slot si's batch on the stream: reset, sieve, Miller-Rabin, stage D and pack,
+// Enqueues slot si's batch on the stream: staging, sieve, Miller-Rabin, stage D and pack,
 // between events ev0 and ev3 (ev1/ev2 bracket the sieve kernel when it is not replayed
 // from a graph). With the graph on, the batch is captured once per slot and launch shape
 // (on a stream where nothing runs) and replayed after that. The persistent grid does not
@@ -1404,7 +1417,8 @@ int DeviceCtx::enqueue_batch(int si) {
   Slot &b = m.slot[si];
   const LaunchCfg &cfg = b.cfg;
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
-    dev::k_reset<<<1, 64, 0, st>>>(b.d_scalars);
+    dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
+                                                        (uint2 *)b.d_tiles, b.d_scalars);
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(b.ev1, st));
     if (cfg.persistent) {
@@ -1499,7 +1513,7 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
       HIP_TRY(hipMalloc(&b.d_tiles, b.tiles_cap * sizeof(TileDesc)));
       if (b.d_tile_counts) HIP_TRY(hipFree(b.d_tile_counts));
       HIP_TRY(hipMalloc(&b.d_tile_counts, b.tiles_cap * sizeof(uint4)));
-      HIP_TRY(pinned_alloc((void **)&b.h_tiles, nullptr, b.tiles_cap * sizeof(TileDesc)));
+      HIP_TRY(pinned_alloc((void **)&b.h_tiles, (void **)&b.dh_tiles, b.tiles_cap * sizeof(TileDesc)));
       HIP_TRY(pinned_alloc((void **)&b.h_tile_counts, (void **)&b.dh_tile_counts, b.tiles_cap * sizeof(uint4)));
       b.graph_ok = false;
     }
@@ -1509,9 +1523,7 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     b.h_hdr->nwin = nwin;
     b.h_hdr->has_sieve = has_sieve ? 1 : 0;
     b.h_hdr->pad = 0;
-    HIP_TRY(hipMemcpyAsync(b.d_hdr, b.h_hdr, sizeof(dev::BatchHdr), hipMemcpyHostToDevice, m.stream));
-    std::memcpy(b.h_tiles, tiles.data(), nt * sizeof(TileDesc));
-    HIP_TRY(hipMemcpyAsync(b.d_tiles, b.h_tiles, nt * sizeof(TileDesc), hipMemcpyHostToDevice, m.stream));
+    std::memcpy(b.h_tiles, tiles.data(), nt * sizeof(TileDesc));   // k_stage copies both in
 
     // One LDS byte per position, sized to the batch's largest tile (rounded to whole
     // 32-position chunks): short tiles then leave room for more resident workgroups.
