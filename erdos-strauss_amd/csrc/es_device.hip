@@ -21,6 +21,7 @@
 //
 // Reference behaviour: sieve/cuda/mark.cu:22-91 (k_mark), survivors.cu:20-133
 // (k_scan_counts, k_compact, k_primality), rung_scan3.cpp:711-784 (sweep_lane).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1408,8 +1409,8 @@ int DeviceCtx::grow_surv(int si, size_t need) {
 }
 
 // Enqueues slot si's batch on the stream: staging, sieve, Miller-Rabin, stage D and pack,
-// between events ev0 and ev3 (ev1/ev2 bracket the sieve kernel when it is not replayed
-// from a graph). With the graph on, the batch is captured once per slot and launch shape
+// between events ev0 and ev3 (ev1/ev2 time the sieve kernel's own dispatch when it is not
+// replayed from a graph). With the graph on, the batch is captured once per slot and launch shape
 // (on a stream where nothing runs) and replayed after that. The persistent grid does not
 // depend on the batch, which is what lets one graph serve every batch that fits.
 int DeviceCtx::enqueue_batch(int si) {
@@ -1420,19 +1421,21 @@ int DeviceCtx::enqueue_batch(int si) {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
                                                         (uint2 *)b.d_tiles, b.d_scalars);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(b.ev1, st));
+    // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
+    // not by separate event packets around it.
+    const hipEvent_t e1 = timed ? b.ev1 : nullptr, e2 = timed ? b.ev2 : nullptr;
     if (cfg.persistent) {
-      dev::k_sieve<true><<<b.grid, cfg.block, b.lds, st>>>(b.d_tiles, b.d_hdr, m.d_lays, b.d_tile_counts,
-                                                           b.d_surv_p, b.d_surv_w, (u32)b.surv_cap,
-                                                           b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
+      hipExtLaunchKernelGGL(dev::k_sieve<true>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
+                            (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
+                            (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
     } else {
-      dev::k_sieve<false><<<(u32)b.nt, cfg.block, b.lds, st>>>(b.d_tiles, b.d_hdr, m.d_lays, b.d_tile_counts,
-                                                               b.d_surv_p, b.d_surv_w, (u32)b.surv_cap,
-                                                               b.d_scalars, b.d_scalars + 1, b.lds_words,
-                                                               b.off_words);
+      hipExtLaunchKernelGGL(dev::k_sieve<false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
+                            (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
+                            (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
     }
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(b.ev2, st));
     dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
                                                            b.d_one);
     HIP_TRY(hipGetLastError());
