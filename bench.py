@@ -155,14 +155,16 @@ def c4_bench(es, reps=5):
             "by_wheel": res}
 
 
-def pipelined(sc, batch, steps, want=None):
-    """steps passes over one batch with two in flight (submit i+1, then collect i): the
-    wall time, and whether every pass's first record equals want (when given)."""
+def pipelined(sc, batch, steps, want=None, depth=4):
+    """steps passes over one batch with up to `depth` in flight (pass i+depth-1 is
+    submitted before pass i is collected): the wall time, and whether every pass's first
+    record equals want (when given)."""
     ok = True
     t0 = time.perf_counter()
-    sc.submit(batch)
+    for i in range(min(depth - 1, steps)):
+        sc.submit(batch)
     for i in range(steps):
-        if i + 1 < steps:
+        if i + depth - 1 < steps:
             sc.submit(batch)
         recs, _ = sc.collect()
         if want is not None:
@@ -190,13 +192,13 @@ def tuned_table_bench(es, block, bits, steps):
         sc.set_graph(True)
         r = {}
         if name == "mi355x":
-            pipelined(sc, [(LO, HI)], 3)
+            pipelined(sc, [(LO, HI)], 4)
             wall, ok = pipelined(sc, [(LO, HI)], steps, gold)
             r["c2"] = {"value": (HI - LO) * steps / wall, "ms_per_step": wall * 1e3 / steps, "record_identical": ok,
                        "device_ms": sc.timing()["device_ms"]}
         recs, _ = sc.records(wins)
         first_ok = name != "mi355x" or recs[0] == gold
-        pipelined(sc, wins, 2)
+        pipelined(sc, wins, 4)
         wall, ok = pipelined(sc, wins, 6, recs[0])
         wall /= 6
         r["batch32"] = {"value": (wins[-1][1] - wins[0][0]) / wall, "ms_per_batch": wall * 1e3,
@@ -240,7 +242,7 @@ def c5_bench(es, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -290,10 +292,10 @@ def main():
         ok_batched = ok_batched and rec == gold
     seq_wall = time.perf_counter() - t0
     timing = sc.timing()
-    # Steady state (the headline): two steps in flight, the host tail and record of step
-    # i overlapping the device work of step i+1, as the CLI runs its chunks. Warmup
-    # captures the second buffer slot's graph.
-    if not pipelined(sc, [(LO, HI)], max(2, args.warmup), gold)[1]:
+    # Steady state (the headline): up to four steps in flight, the host tail and record
+    # of step i overlapping the device work of the next steps, as the CLI runs its chunks.
+    # Warmup captures every buffer slot's graph.
+    if not pipelined(sc, [(LO, HI)], max(es.MAX_IN_FLIGHT, args.warmup), gold)[1]:
         raise SystemExit("pipelined record differs from the golden record")
     wall, ok = pipelined(sc, [(LO, HI)], args.steps, gold)
     ok_batched = ok_batched and ok
@@ -350,8 +352,8 @@ def main():
         "config": {"workload": "plain sieve at height 1e15: [1e15, 1e15+2^34), wheel 2042040, "
                                "f_M2042040_mmax2000_gpu.txt (3745 filters)",
                    "form": "batched: one persistent work-queue launch per step replayed from a hipGraph, "
-                           "layout and launch shape autotuned; two steps in flight (the host tail of "
-                           "step i overlaps the device work of step i+1)", "layout_Mp": layout,
+                           "layout and launch shape autotuned; up to four steps in flight, one stream "
+                           "each (the host tail of step i overlaps the device work of the next steps)", "layout_Mp": layout,
                    "tiles": work["tiles"], "block": block, "tile_bits": bits},
         "record_identical": ok_batched,
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},

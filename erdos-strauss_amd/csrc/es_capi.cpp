@@ -112,9 +112,11 @@ int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_
   return records_text(s, recs, emit_survivors, text, text_len, rec_len, exit_code);
 }
 
+static_assert(ES_MAX_IN_FLIGHT == es::DeviceCtx::kMaxInFlight, "header and device agree");
+
 int es_scan_submit(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin) {
   if (!s || (nwin && (!lo || !hi))) return fail(ES_ERR_ARG, "null argument");
-  if (s->sc.in_flight() >= 2) return fail(ES_ERR_ARG, "two batches are already in flight: collect one first");
+  if (s->sc.in_flight() >= ES_MAX_IN_FLIGHT) return fail(ES_ERR_ARG, "every batch slot is in flight: collect one first");
   std::vector<es::Interval> w(nwin);
   std::string msg;
   for (size_t i = 0; i < nwin; ++i) {

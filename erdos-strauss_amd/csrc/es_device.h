@@ -74,12 +74,13 @@ class DeviceCtx {
   // valid layouts/lanes, nbits <= cfg.max_bits.
   int run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg,
           BatchResult &out);
-  // Pipelined form: launch enqueues a batch and returns at once (at most two in flight);
+  // Pipelined form: launch enqueues a batch and returns at once (at most kMaxInFlight);
   // finish waits for the oldest batch in flight and returns its results. The host can
   // finish one batch while the device runs the next.
   int launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg);
   int finish(BatchResult &out);
   int in_flight() const;
+  static constexpr int kMaxInFlight = 4;
   // Deterministic MR of arbitrary values on the device (parity checks).
   int primality(const u64 *vals, size_t n, uint8_t *flags);
   // Stage D on the device for primes vals[0..n) (one RungResult each).

@@ -1156,11 +1156,19 @@ struct DevLayoutBufs {
        *fc = nullptr, *off = nullptr;
 };
 
-// One batch's buffers, captured graph and events. Two slots let the host plan and launch
-// batch i+1 while the device still runs batch i, then finish batch i (mirrors, stage-D
-// leftovers, records) while batch i+1 runs: the overlap of the reference driver
-// (sieve/cuda/scan.cu:1-6, 197-207), over whole batches.
+// One batch's buffers, captured graph and events. Several slots let the host plan and
+// launch batch i+1 while the device still runs batch i, then finish batch i (mirrors,
+// stage-D leftovers, records) while later batches run: the overlap of the reference driver
+// (sieve/cuda/scan.cu:1-6, 197-207), over whole batches. Further slots keep batches
+// queued while the host finishes one and submits the next, so the device does not idle
+// behind the host. Four slots, one stream each, match the four hardware queues HIP gives
+// a process by default; with more streams than queues, streams share a queue and their
+// batches serialise (measured: six slots ran C2 steps at 90 us against four slots' 80).
 struct Slot {
+  // Each slot has its own stream, so the device may also overlap batches: one batch's
+  // stage D (a few primes on a few CUs) with the next batch's staging and sieve.
+  // The batches share only read-only tables, and no kernel waits on another workgroup.
+  hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -1206,7 +1214,7 @@ struct Slot {
 };
 
 struct DeviceCtx::Impl {
-  hipStream_t stream = nullptr;     // every batch, in submission order
+  hipStream_t stream = nullptr;     // slot 0's stream; also the one-off calls (rung, primality, probes)
   hipStream_t capture = nullptr;    // graph capture only: nothing ever runs on it
   hipEvent_t pev0 = nullptr, pev1 = nullptr;   // the LDS probes' timing
   std::vector<DevLayoutBufs> layouts;
@@ -1217,9 +1225,9 @@ struct DeviceCtx::Impl {
   u32 occ_block = 0;
   size_t occ_lds = 0;
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
-  Slot slot[2];
+  Slot slot[DeviceCtx::kMaxInFlight];
   int head = 0;                     // the oldest batch in flight
-  int busy = 0;                     // batches in flight (0, 1 or 2)
+  int busy = 0;                     // batches in flight (0 .. kMaxInFlight)
 };
 
 DeviceCtx::DeviceCtx() : impl_(new Impl) {}
@@ -1229,7 +1237,8 @@ DeviceCtx::~DeviceCtx() {
   Impl &m = *impl_;
   if (device_ >= 0) {
     (void)hipSetDevice(device_);
-    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    for (Slot &b : m.slot)
+      if (b.stream) (void)hipStreamSynchronize(b.stream);
     for (DevLayoutBufs &b : m.layouts)
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
     for (void *p : {(void *)m.d_lays, (void *)m.trial.qm, (void *)m.trial.inv, (void *)m.trial.lim})
@@ -1248,7 +1257,9 @@ DeviceCtx::~DeviceCtx() {
     }
     for (hipEvent_t e : {m.pev0, m.pev1}) if (e) (void)hipEventDestroy(e);
     if (m.capture) (void)hipStreamDestroy(m.capture);
-    if (m.stream) (void)hipStreamDestroy(m.stream);
+    for (int i = 1; i < kMaxInFlight; ++i)
+      if (m.slot[i].stream) (void)hipStreamDestroy(m.slot[i].stream);
+    if (m.stream) (void)hipStreamDestroy(m.stream);   // slot 0's
   }
   delete impl_;
 }
@@ -1271,6 +1282,8 @@ int DeviceCtx::open(int device) {
   lds_limit_ = p.sharedMemPerBlock;
   Impl &m = *impl_;
   HIP_TRY(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+  m.slot[0].stream = m.stream;
+  for (int i = 1; i < kMaxInFlight; ++i) HIP_TRY(hipStreamCreateWithFlags(&m.slot[i].stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&m.capture, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&m.pev0));
   HIP_TRY(hipEventCreate(&m.pev1));
@@ -1455,7 +1468,7 @@ int DeviceCtx::enqueue_batch(int si) {
     HIP_TRY(hipGetLastError());
     return 0;
   };
-  HIP_TRY(hipEventRecord(b.ev0, m.stream));
+  HIP_TRY(hipEventRecord(b.ev0, b.stream));
   if (cfg.graph && cfg.persistent) {
     if (!b.graph_ok || b.graph_block != cfg.block || b.graph_lds != b.lds || b.graph_tail != cfg.device_tail ||
         b.graph_grid != b.grid) {
@@ -1475,18 +1488,18 @@ int DeviceCtx::enqueue_batch(int si) {
       b.graph_tail = cfg.device_tail;
       b.graph_grid = b.grid;
     }
-    HIP_TRY(hipGraphLaunch(b.graph_exec, m.stream));
-  } else if (enqueue(m.stream, true)) {
+    HIP_TRY(hipGraphLaunch(b.graph_exec, b.stream));
+  } else if (enqueue(b.stream, true)) {
     return -1;
   }
-  HIP_TRY(hipEventRecord(b.ev3, m.stream));
+  HIP_TRY(hipEventRecord(b.ev3, b.stream));
   return 0;
 }
 
 int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
-  if (m.busy == 2) { err_ = "two batches are already in flight"; return -1; }
+  if (m.busy == kMaxInFlight) { err_ = "every batch slot is in flight"; return -1; }
   HIP_TRY(hipSetDevice(device_));
   const size_t nt = tiles.size();
   if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
@@ -1503,7 +1516,7 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     }
   }
   if (nt > 0xFFFFFFFFull || nwin > 0x55555555u) { err_ = "batch too large"; return -1; }
-  const int si = (m.head + m.busy) & 1;
+  const int si = (m.head + m.busy) % kMaxInFlight;
   Slot &b = m.slot[si];
   b.nt = nt;
   b.nwin = nwin;
@@ -1571,7 +1584,7 @@ int DeviceCtx::finish(BatchResult &out) {
   Slot &b = m.slot[si];
   // The slot is released whatever happens below. With nothing left in flight the next
   // batch takes slot 0 again, so one-at-a-time use keeps to one slot and one graph.
-  m.head ^= 1;
+  m.head = (m.head + 1) % kMaxInFlight;
   if (--m.busy == 0) m.head = 0;
   out.counts.assign(b.nwin, WinCounts{0, 0, 0});
   out.surv_p.clear();

@@ -467,7 +467,7 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
 }
 
 int Scanner::submit(const std::vector<Interval> &wins, std::string &msg) {
-  if (pending_.size() >= 2) { msg = "FATAL: two batches are already in flight"; return -1; }
+  if (pending_.size() >= (size_t)DeviceCtx::kMaxInFlight) { msg = "FATAL: every batch slot is in flight"; return -1; }
   return device_submit(wins, msg);
 }
 

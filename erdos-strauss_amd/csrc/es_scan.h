@@ -62,8 +62,8 @@ class Scanner {
   // -1 on a device error (msg).
   int scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, ScanTiming &tm,
            std::string &msg);
-  // Pipelined scan: submit plans a batch and launches it without waiting (at most two
-  // batches in flight); collect finishes the oldest submitted batch into the records
+  // Pipelined scan: submit plans a batch and launches it without waiting (at most
+  // DeviceCtx::kMaxInFlight in flight); collect finishes the oldest submitted batch into the records
   // scan would have produced. The host tail of one batch then runs while the device
   // works on the next (sieve/cuda/scan.cu:1-6, 197-207). scan refuses while a batch is
   // in flight.

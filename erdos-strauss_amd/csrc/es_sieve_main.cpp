@@ -404,8 +404,8 @@ int main(int argc, char **argv) {
       if (o.emit_surv || l.text.compare(0, 9, "SURVIVOR ") != 0) fwrite(l.text.data(), 1, l.text.size(), stdout);
     all.clear();
   };
-  // Chunk k+1 is submitted before chunk k is collected, so the host tail of one chunk
-  // overlaps the device work of the next (sieve/cuda/scan.cu:1-6).
+  // Chunks k+1 .. k+kMaxInFlight-1 are submitted before chunk k is collected, so the host
+  // tail of one chunk overlaps the device work of the next ones (sieve/cuda/scan.cu:1-6).
   std::deque<Interval> inflight;
   u64 next = o.lo;
   auto submit_next = [&]() -> bool {
@@ -415,7 +415,8 @@ int main(int argc, char **argv) {
     next = b;
     return true;
   };
-  if (next < o.hi && !submit_next()) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
+  while (next < o.hi && inflight.size() + 1 < (size_t)DeviceCtx::kMaxInFlight)
+    if (!submit_next()) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
   while (!inflight.empty()) {
     if (next < o.hi && !submit_next()) { fprintf(stderr, "%s\n", msg.c_str()); return 3; }
     const u64 a = inflight.front().lo, b = inflight.front().hi;

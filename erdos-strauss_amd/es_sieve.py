@@ -152,6 +152,9 @@ def _u64s(vals):
     return (U64 * len(vals))(*vals)
 
 
+MAX_IN_FLIGHT = 4   # ES_MAX_IN_FLIGHT of include/es_sieve.h
+
+
 class Scanner:
     """One scanner configuration resident on a HIP device (rung_scan3's filter flags)."""
 
@@ -212,7 +215,8 @@ class Scanner:
 
     def submit(self, windows):
         """Pipelined records(): plans [(lo, hi)] and launches the batch without waiting.
-        At most two batches are in flight; collect() returns them in submission order."""
+        At most MAX_IN_FLIGHT batches are in flight; collect() returns them in submission
+        order."""
         n = len(windows)
         _check(lib().es_scan_submit(self.h, _u64s([w[0] for w in windows]), _u64s([w[1] for w in windows]), n))
         self._sizes.append(n)

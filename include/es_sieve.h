@@ -84,15 +84,17 @@ int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_
 /* Pipelined es_scan_records. es_scan_submit plans the windows and launches their batch
  * without waiting; es_scan_collect finishes the oldest submitted batch and returns its
  * records exactly as es_scan_records would have (rec_len and exit_code need room for that
- * batch's window count, which *nwin receives). At most two batches are in flight, so the
- * host tail of one (stage-D leftovers, record text) runs while the device works on the
- * next: the overlap of sieve_cuda's driver, which keeps round s on the host tail while
- * round s+1 runs on the device (sieve/cuda/scan.cu:1-6, 197-207). es_scan_records,
- * es_scan_counts and es_survivors refuse (ES_ERR_ARG) while a batch is in flight. */
+ * batch's window count, which *nwin receives). Up to ES_MAX_IN_FLIGHT batches are in
+ * flight (a further submit returns ES_ERR_ARG), so the host tail of one (stage-D
+ * leftovers, record text) runs while the device works on the next: the overlap of
+ * sieve_cuda's driver, which keeps round s on the host tail while round s+1 runs on the
+ * device (sieve/cuda/scan.cu:1-6, 197-207). es_scan_records, es_scan_counts,
+ * es_survivors and es_autotune refuse (ES_ERR_ARG) while a batch is in flight. */
+#define ES_MAX_IN_FLIGHT 4
 int es_scan_submit(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin);
 int es_scan_collect(es_scanner *s, int emit_survivors, char **text, size_t *text_len,
                     size_t *rec_len, int *exit_code, size_t *nwin);
-/* Batches submitted and not yet collected (0, 1 or 2). */
+/* Batches submitted and not yet collected (0 .. ES_MAX_IN_FLIGHT). */
 int es_scan_in_flight(const es_scanner *s);
 
 /* As es_scan_records, counters only. */

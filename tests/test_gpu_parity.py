@@ -202,12 +202,13 @@ def test_graph_replays_match_eager():
         sc.set_graph(False)
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_pipelined_batches_match_records(graph):
-    """submit/collect with two batches in flight (the host tail of one batch overlapping
-    the device work of the next) returns, batch by batch and in submission order, the
-    records records() gives one batch at a time: uneven batches, a batch with an empty
-    window, an empty batch, each slot reused several times, graph on and off."""
+@pytest.mark.parametrize("graph,depth", [(False, 2), (True, 2), (False, 4), (True, 4)])
+def test_pipelined_batches_match_records(graph, depth):
+    """submit/collect with up to `depth` batches in flight (the host tail of one batch
+    overlapping the device work of the next ones, each slot on its own stream) returns,
+    batch by batch and in submission order, the records records() gives one batch at a
+    time: uneven batches, a batch with an empty window, an empty batch, each slot reused
+    several times, graph on and off."""
     kw = dict(wheel=2042040, mmax=10000, sieve=0, filters=F204)
     sc = scanner(**kw)
     batches = [[(10 ** 15, 10 ** 15 + 2 ** 30)], [(10 ** 13, 10 ** 13 + 10 ** 9), (10 ** 16, 10 ** 16)],
@@ -217,12 +218,13 @@ def test_pipelined_batches_match_records(graph):
     sc.set_graph(graph)
     try:
         for rep in range(2):
-            print("pipelined", graph, rep, flush=True)
-            sc.submit(batches[0])
+            print("pipelined", graph, depth, rep, flush=True)
+            for b in batches[:depth - 1]:
+                sc.submit(b)
             for i in range(len(batches)):
-                if i + 1 < len(batches):
-                    sc.submit(batches[i + 1])
-                assert sc.in_flight() == (2 if i + 1 < len(batches) else 1)
+                if i + depth - 1 < len(batches):
+                    sc.submit(batches[i + depth - 1])
+                assert sc.in_flight() == min(depth, len(batches) - i)
                 assert sc.collect() == want[i], (rep, i)
             assert sc.in_flight() == 0
     finally:
@@ -230,21 +232,22 @@ def test_pipelined_batches_match_records(graph):
 
 
 def test_pipelined_refusals():
-    """At most two batches in flight; no collect without a submission; the one-shot entry
-    points refuse while a batch is in flight and work again once it is collected."""
+    """At most MAX_IN_FLIGHT batches in flight; no collect without a submission; the
+    one-shot entry points refuse while a batch is in flight and work again once it is
+    collected."""
     sc = scanner(wheel=2042040, mmax=10000, sieve=0, filters=F204)
     w = [(10 ** 13, 10 ** 13 + 2 ** 24)]
     want = sc.records(w)
     with pytest.raises(es.EsError):
         sc.collect()
-    sc.submit(w)
-    sc.submit(w)
+    for _ in range(es.MAX_IN_FLIGHT):
+        sc.submit(w)
     with pytest.raises(es.EsError):
         sc.submit(w)
     with pytest.raises(es.EsError):
         sc.records(w)
-    assert sc.collect() == want
-    assert sc.collect() == want
+    for _ in range(es.MAX_IN_FLIGHT):
+        assert sc.collect() == want
     assert sc.records(w) == want
 
 
@@ -381,7 +384,7 @@ def test_cli_round_order_across_chunks():
 
 def test_cli_lane_order_across_chunks():
     """Lane order holds every chunk's lines to the end: the chunked CLI run (eight
-    chunks, pipelined two deep) gives the record of the whole window in one batch."""
+    chunks, pipelined) gives the record of the whole window in one batch."""
     with es.Scanner(wheel=2042040, filters=F204, device=0) as sc:
         (want,), (ec,) = sc.records([(W19_LO, W19_HI)])
     argv = [str(W19_LO), str(W19_HI), "--filters", os.path.relpath(F204, paths.REPO), "--chunk", str(1 << 37)]
