@@ -136,6 +136,11 @@ def c4_bench(es, reps=5):
             ts.append(time.perf_counter() - t)
             ok = ok and recs == gold
         tm = sc.timing()
+        # The same batch repeated with up to four in flight (the host records of one batch
+        # beside the device work of the next).
+        pipelined(sc, wins, 4)
+        pwall, pok = pipelined(sc, wins, 2 * reps, gold[0])
+        ok = ok and pok
         sc.set_graph(False)
         block, bits = tuned_launch(report)
         sc.set_launch(False, block, bits)
@@ -144,7 +149,9 @@ def c4_bench(es, reps=5):
         plain_s = time.perf_counter() - t
         ok_plain = precs == gold
         bt = sorted(ts)[len(ts) // 2]
-        res[M] = {"batched": total / bt, "batched_ms": bt * 1e3, "plain": total / plain_s,
+        res[M] = {"batched": total / bt, "batched_ms": bt * 1e3,
+                  "batched_pipelined": total * 2 * reps / pwall, "batched_pipelined_ms": pwall * 1e3 / (2 * reps),
+                  "plain": total / plain_s,
                   "plain_ms": plain_s * 1e3, "batched_over_plain": plain_s / bt,
                   "records_identical": ok and ok_plain, "tiles": tm["tiles"], "block": block,
                   "tile_bits": bits, "device_ms": tm["device_ms"], "tail_ms": tm["tail_ms"],
