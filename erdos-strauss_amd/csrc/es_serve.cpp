@@ -158,75 +158,21 @@ void Server::finish(Req &r) {
 }
 
 void Server::loop() {
+  // Up to DeviceCtx::kMaxInFlight batches are on the device at once: the worker builds and
+  // submits the next batch (admitting whatever arrived meanwhile) before it collects the
+  // oldest, so the slices of one request run in consecutive batches side by side.
+  struct Flight {
+    std::vector<Interval> wins;
+    std::vector<u64> owner;
+    u64 ints = 0;
+    double t0 = 0;
+  };
   std::unique_lock<std::mutex> lk(mu_);
-  std::vector<Interval> wins;
-  std::vector<u64> owner;
+  std::deque<Flight> flight;
   std::vector<WindowRecord> recs;
   for (;;) {
-    // Admission: FIFO while a slot and the budget allow (one request always fits, so an
-    // oversized one runs alone rather than never).
-    while (!queue_.empty() && active_.size() < cfg_.slots) {
-      Req &r = reqs_[queue_.front()];
-      if (!active_.empty() && st_.reserved + r.info.reserved > avail_) break;
-      st_.reserved += r.info.reserved;
-      st_.peak_reserved = std::max(st_.peak_reserved, st_.reserved);
-      r.info.admit_ms = now_ms();
-      active_.push_back(queue_.front());
-      queue_.pop_front();
-    }
-    st_.max_active = std::max<u32>(st_.max_active, (u32)active_.size());
-    if (active_.empty()) {
-      if (stop_ || st_.error) break;
-      cv_work_.wait(lk);
-      continue;
-    }
-    // One batch: round robin over the admitted requests, one slice each per pass.
-    wins.clear();
-    owner.clear();
-    u64 ints = 0;
-    for (bool more = true; more && wins.size() < cfg_.batch_windows;) {
-      more = false;
-      for (u64 id : active_) {
-        Req &r = reqs_[id];
-        if (r.next >= r.hi) continue;
-        const u64 len = std::min(cfg_.slice, r.hi - r.next);
-        if (!wins.empty() && ints + len > cfg_.batch_ints) continue;
-        wins.push_back(Interval{r.next, r.next + len});
-        owner.push_back(id);
-        r.next += len;
-        ints += len;
-        more = true;
-        if (wins.size() >= cfg_.batch_windows) break;
-      }
-    }
-    std::vector<u64> in_batch(owner);
-    std::sort(in_batch.begin(), in_batch.end());
-    in_batch.erase(std::unique(in_batch.begin(), in_batch.end()), in_batch.end());
-    lk.unlock();
-    ScanTiming tm;
-    std::string msg;
-    const double b0 = clock_ms();
-    recs.clear();
-    const int rc = wins.empty() ? 0 : sc_.scan(wins, recs, tm, msg);   // empty requests only
-    const double b1 = clock_ms();
-    lk.lock();
-    st_.busy_ms += b1 - b0;
-    if (rc) {
-      st_.error = 1;
-      st_.error_msg = msg;
-      cv_done_.notify_all();
-      break;
-    }
-    if (!wins.empty()) ++st_.batches;
-    st_.slices += wins.size();
-    st_.ints += ints;
-    st_.max_batch_requests = std::max<u32>(st_.max_batch_requests, (u32)in_batch.size());
-    for (u64 id : in_batch) ++reqs_[id].info.batches;
-    for (size_t i = 0; i < wins.size(); ++i) {
-      Req &r = reqs_[owner[i]];
-      r.done_ints += wins[i].hi - wins[i].lo;
-      r.parts.push_back(std::move(recs[i]));
-    }
+    // Requests with nothing left in flight or to issue (empty ones included) complete,
+    // which frees their slots and budget for the admission below.
     bool any = false;
     for (size_t i = 0; i < active_.size();) {
       Req &r = reqs_[active_[i]];
@@ -241,6 +187,112 @@ void Server::loop() {
       }
     }
     if (any) cv_done_.notify_all();
+    // Admission: FIFO while a slot and the budget allow (one request always fits, so an
+    // oversized one runs alone rather than never).
+    while (!queue_.empty() && active_.size() < cfg_.slots) {
+      Req &r = reqs_[queue_.front()];
+      if (!active_.empty() && st_.reserved + r.info.reserved > avail_) break;
+      st_.reserved += r.info.reserved;
+      st_.peak_reserved = std::max(st_.peak_reserved, st_.reserved);
+      r.info.admit_ms = now_ms();
+      active_.push_back(queue_.front());
+      queue_.pop_front();
+    }
+    st_.max_active = std::max<u32>(st_.max_active, (u32)active_.size());
+    // An admitted empty request completes at once.
+    for (size_t i = 0; i < active_.size();) {
+      Req &r = reqs_[active_[i]];
+      if (r.hi == r.lo) {
+        finish(r);
+        st_.reserved -= r.info.reserved;
+        ++st_.completed;
+        active_.erase(active_.begin() + i);
+        cv_done_.notify_all();
+      } else {
+        ++i;
+      }
+    }
+    bool issuable = false;
+    for (u64 id : active_) issuable |= reqs_[id].next < reqs_[id].hi;
+    if (issuable && flight.size() < (size_t)DeviceCtx::kMaxInFlight) {
+      // One batch: round robin over the admitted requests, one slice each per pass.
+      Flight f;
+      for (bool more = true; more && f.wins.size() < cfg_.batch_windows;) {
+        more = false;
+        for (u64 id : active_) {
+          Req &r = reqs_[id];
+          if (r.next >= r.hi) continue;
+          const u64 len = std::min(cfg_.slice, r.hi - r.next);
+          if (!f.wins.empty() && f.ints + len > cfg_.batch_ints) continue;
+          f.wins.push_back(Interval{r.next, r.next + len});
+          f.owner.push_back(id);
+          r.next += len;
+          f.ints += len;
+          more = true;
+          if (f.wins.size() >= cfg_.batch_windows) break;
+        }
+      }
+      std::vector<u64> in_batch(f.owner);
+      std::sort(in_batch.begin(), in_batch.end());
+      in_batch.erase(std::unique(in_batch.begin(), in_batch.end()), in_batch.end());
+      st_.max_batch_requests = std::max<u32>(st_.max_batch_requests, (u32)in_batch.size());
+      for (u64 id : in_batch) ++reqs_[id].info.batches;
+      lk.unlock();
+      std::string msg;
+      f.t0 = clock_ms();
+      const int rc = sc_.submit(f.wins, msg);
+      lk.lock();
+      if (rc) {
+        st_.error = 1;
+        st_.error_msg = msg;
+        cv_done_.notify_all();
+        break;
+      }
+      flight.push_back(std::move(f));
+      continue;   // admit and submit again before collecting, while slots remain
+    }
+    if (flight.empty()) {
+      // Nothing on the device and nothing to issue: every admitted request has completed
+      // above, so the loop sleeps until a request arrives (or stop), unless one is queued.
+      if (active_.empty() && queue_.empty()) {
+        if (stop_ || st_.error) break;
+        cv_work_.wait(lk);
+      }
+      continue;
+    }
+    // Collect the oldest batch in flight.
+    Flight f = std::move(flight.front());
+    flight.pop_front();
+    lk.unlock();
+    ScanTiming tm;
+    std::string msg;
+    recs.clear();
+    const int rc = sc_.collect(recs, tm, msg);
+    const double b1 = clock_ms();
+    lk.lock();
+    st_.busy_ms += b1 - f.t0;
+    if (rc) {
+      st_.error = 1;
+      st_.error_msg = msg;
+      cv_done_.notify_all();
+      break;
+    }
+    ++st_.batches;
+    st_.slices += f.wins.size();
+    st_.ints += f.ints;
+    for (size_t i = 0; i < f.wins.size(); ++i) {
+      Req &r = reqs_[f.owner[i]];
+      r.done_ints += f.wins[i].hi - f.wins[i].lo;
+      r.parts.push_back(std::move(recs[i]));
+    }
+  }
+  // A failure leaves batches in flight; collect them so the scanner is usable again.
+  lk.unlock();
+  for (size_t k = 0; k < flight.size(); ++k) {
+    ScanTiming tm;
+    std::string msg;
+    recs.clear();
+    (void)sc_.collect(recs, tm, msg);
   }
 }
 

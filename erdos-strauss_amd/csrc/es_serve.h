@@ -2,14 +2,15 @@
 //
 // Requests are intervals [lo, hi) whose answer is exactly the record rung_scan3 prints
 // for `rung_scan3 lo hi` with the scanner's wheel and filters (rung_scan3.cpp:1128-1162).
-// A worker thread owns the scanner and runs device batches back to back:
+// A worker thread owns the scanner and keeps up to four device batches in flight:
 //   * admission: FIFO, bounded by request slots and by an HBM budget (a fraction of the
 //     device's memory, minus what the scanner keeps resident); each request reserves an
 //     upper bound of the device memory its slices occupy until it completes;
 //   * batching: every batch takes the next slice (<= `slice` integers) of each admitted
 //     request, round robin, up to `batch_ints` integers / `batch_windows` slices, and
-//     runs them as one persistent launch -- requests that arrive while a batch is on the
-//     device join the next one, so short requests never wait behind long ones;
+//     runs them as one persistent launch -- requests that arrive while batches are on the
+//     device join the next one, so short requests never wait behind long ones, and the
+//     slices of a long request run in consecutive batches side by side;
 //   * completion: a request's slice records merge into one record (lines in the
 //     scanner's emission order over the whole request, counters summed), which is what
 //     the same request gives alone.
@@ -47,7 +48,7 @@ struct ServeStats {
   u64 submitted = 0, completed = 0, refused = 0, batches = 0, slices = 0, ints = 0;
   u64 budget = 0, resident = 0, reserved = 0, peak_reserved = 0;
   u32 max_active = 0, max_batch_requests = 0, max_queued = 0;
-  double busy_ms = 0;              // time spent inside device batches
+  double busy_ms = 0;              // summed submit-to-collect time of the device batches
   int error = 0;                   // nonzero after a device failure (message in error_msg)
   std::string error_msg;
 };
