@@ -102,6 +102,7 @@ class DeviceCtx {
   Impl *impl_ = nullptr;
   int grow_surv(int slot, size_t need);
   int enqueue_batch(int slot);
+  int read_batch(int slot, BatchResult &out);
   std::string err_;
   int device_ = -1;
   int cus_ = 0;

@@ -1579,13 +1579,21 @@ int DeviceCtx::finish(BatchResult &out) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
   if (m.busy == 0) { err_ = "no batch in flight"; return -1; }
-  HIP_TRY(hipSetDevice(device_));
-  const int si = m.head;
-  Slot &b = m.slot[si];
-  // The slot is released whatever happens below. With nothing left in flight the next
-  // batch takes slot 0 again, so one-at-a-time use keeps to one slot and one graph.
+  const int rc = read_batch(m.head, out);
+  // The slot is released only now, once its mirrors have been read, and whatever the
+  // outcome. With nothing left in flight the next batch takes slot 0 again, so
+  // one-at-a-time use keeps to one slot and one graph.
   m.head = (m.head + 1) % kMaxInFlight;
   if (--m.busy == 0) m.head = 0;
+  return rc;
+}
+
+// Waits for slot si's batch (only that batch: later ones keep running), replays it on a
+// survivor-list overflow, and copies its results out of the pinned mirrors.
+int DeviceCtx::read_batch(int si, BatchResult &out) {
+  Impl &m = *impl_;
+  HIP_TRY(hipSetDevice(device_));
+  Slot &b = m.slot[si];
   out.counts.assign(b.nwin, WinCounts{0, 0, 0});
   out.surv_p.clear();
   out.surv_w.clear();
@@ -1596,7 +1604,6 @@ int DeviceCtx::finish(BatchResult &out) {
   if (b.nt == 0) return 0;
   const bool use_graph = b.cfg.graph && b.cfg.persistent;
   for (;;) {
-    // Waits for this batch only: a later batch keeps running meanwhile.
     HIP_TRY(hipEventSynchronize(b.ev3));
     float all = 0, sv = 0;
     HIP_TRY(hipEventElapsedTime(&all, b.ev0, b.ev3));
