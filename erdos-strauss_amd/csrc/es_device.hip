@@ -1375,6 +1375,8 @@ int DeviceCtx::add_layout(const Layout &L) {
   }
   // Mirror the descriptor array on the device.
   if (m.layouts.size() > m.lays_cap) {
+    // Batches in flight read the old array: let them finish before it goes.
+    for (Slot &b : m.slot) HIP_TRY(hipStreamSynchronize(b.stream));
     if (m.d_lays) HIP_TRY(hipFree(m.d_lays));
     m.lays_cap = std::max<size_t>(8, m.layouts.size() * 2);
     HIP_TRY(hipMalloc(&m.d_lays, m.lays_cap * sizeof(dev::LayoutDev)));
