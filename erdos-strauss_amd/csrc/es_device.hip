@@ -1,6 +1,8 @@
 // es_device.hip -- gfx950 kernels of the verification-sieve funnel and the host code
 // that drives them.
 //
+//   k_stage   first kernel of a batch: zeroes its counters and queue heads and copies its
+//             header and tile descriptors in from mapped pinned host memory.
 //   k_sieve   one tile = one LDS byte map of <= max_bits positions of one lane. Init
 //             expands the lane's periodic wheel mask (sub-wheel layouts) and the tail
 //             mask to one byte per position; every filter then clears its positions
@@ -18,6 +20,9 @@
 //             r = 3 (mod 4) up to 255 with a small-prime sieve, least-witness search and
 //             Pollard rho, one persistent workgroup per CU taking survivors from a queue.
 //   k_pack    survivor records and tile counters into pinned host memory.
+//
+// Host side: DeviceCtx keeps up to four batches in flight, each in its own slot (buffers,
+// pinned mirrors, captured hipGraph, events) on its own stream.
 //
 // Reference behaviour: sieve/cuda/mark.cu:22-91 (k_mark), survivors.cu:20-133
 // (k_scan_counts, k_compact, k_primality), rung_scan3.cpp:711-784 (sweep_lane).
