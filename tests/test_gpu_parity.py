@@ -373,6 +373,38 @@ def test_round_major_order_reproduces_published_window():
         assert lane != want and sorted(lane.splitlines()) == sorted(want.splitlines())
 
 
+SEG = 262144
+M204 = 2042040
+SHARD_EDGES = [  # published re-runs of production shards: (name, lo, hi)
+    ("s0", 255255, 40000000000254234),
+    ("s69", 15590940119293459447, 15631162710079743316),
+]
+
+
+@pytest.mark.parametrize("name,lo,hi", SHARD_EDGES, ids=[e[0] for e in SHARD_EDGES])
+def test_published_shard_edges(name, lo, hi):
+    """The first and last 2,000 certificate lines of two published production shards
+    (round-major, seg 262144, as sieve_cuda emitted them). Round r holds positions
+    J in [r*seg, (r+1)*seg) of every lane, i.e. the integers [lo + r*seg*M, lo + (r+1)*seg*M),
+    so the head is round 0 scanned alone and the tail is the last two rounds scanned from
+    their boundary: the sieve, stage D and the round key all have to agree with the
+    reference for the lines to come out identical."""
+    head = open(os.path.join(paths.GOLDEN, "rederive", "%s_head.txt" % name)).read().splitlines(keepends=True)
+    tail = open(os.path.join(paths.GOLDEN, "rederive", "%s_tail.txt" % name)).read().splitlines(keepends=True)
+    last_round = ((hi - lo + M204 - 1) // M204 - 1) // SEG
+    with es.Scanner(wheel=M204, filters=F204, device=0) as sc:
+        sc.set_order("round", SEG)
+        print(name, "head", flush=True)
+        (rec,), _ = sc.records([(lo, min(hi, lo + SEG * M204))])
+        assert rec.splitlines(keepends=True)[:2000] == head
+        start = lo + max(0, last_round - 1) * SEG * M204
+        print(name, "tail from", start, flush=True)
+        (rec,), _ = sc.records([(start, hi)])
+        lines = rec.splitlines(keepends=True)
+        assert lines[-1].startswith("SUMMARY ")
+        assert lines[-2001:-1] == tail
+
+
 def test_cli_round_order_across_chunks():
     """The CLI cuts the window into chunks and merges by the key of the whole interval."""
     argv = [str(W19_LO), str(W19_HI), "--filters", os.path.relpath(F204, paths.REPO), "--order", "round",
