@@ -7,7 +7,7 @@ drop-in: the same positional arguments, the same shard arithmetic (the scanner's
 
     run_shards.py [LO HI N [OUTDIR]] [--bin es_sieve] [--table FILE] [--threads 12]
                   [--order round|lane] [--attempts 3] [--retry-sleep 30] [--device D]
-                  [--no-gpu-check]
+                  [--no-gpu-check] [--only I[,J...]]
 
 Each shard's certificate stream is hashed while the scanner writes it and then dropped
 (at 10^19 the stream is terabytes); the meta keeps what makes the shard re-derivable
@@ -120,6 +120,8 @@ def main(argv=None):
     ap.add_argument("--device", type=int, default=None)
     ap.add_argument("--no-gpu-check", action="store_true",
                     help="skip the AMD GPU check (for a CPU scanner with the same command line)")
+    ap.add_argument("--only", default=None, metavar="I[,J...]",
+                    help="run only these shard indices (e.g. to re-derive published shards one at a time)")
     a = ap.parse_args(argv)
     pos = a.pos + [None] * (4 - len(a.pos))
     lo = int(pos[0]) if pos[0] else 255255
@@ -147,8 +149,13 @@ def main(argv=None):
     gitc = commit_id()
     run.log("start: [%d,%d) x%d table=%s commit=%s" % (lo, hi, n, tsha, gitc))
 
+    only = None if a.only is None else {int(x) for x in a.only.split(",") if x != ""}
+    if only is not None and any(not 0 <= i < n for i in only):
+        return run.fatal("--only: shard index out of range [0, %d)" % n)
     work_s, done_now = 0.0, 0
     for i in range(n):
+        if only is not None and i not in only:
+            continue
         meta = os.path.join(a.outdir, "shard_%d.meta" % i)
         if os.path.exists(meta) and any(l.startswith("SUMMARY ") for l in open(meta)):
             continue
