@@ -163,3 +163,19 @@ def test_driver_retry_rules(tmp_path, monkeypatch, mode, meta_rc, attempt, faile
 def test_driver_refuses_missing_configuration(tmp_path):
     p = drive(str(tmp_path / "run"), bin_path=str(tmp_path / "nope"))
     assert p.returncode == 1 and b"FATAL scanner missing" in p.stdout
+
+
+def test_driver_only_runs_the_named_shards(tmp_path):
+    """--only I,J runs exactly those shards (each meta equal to the reference run's), and
+    refuses an index outside [0, N)."""
+    out = str(tmp_path / "run")
+    p = drive(out, "--order", "lane", "--only", "1,5", bin_path=paths.ORACLE_BIN)
+    assert p.returncode == 0, p.stderr.decode()
+    assert sorted(f for f in os.listdir(out) if f.endswith(".meta")) == ["shard_1.meta", "shard_5.meta"]
+    for i in (1, 5):
+        got = audit_shards.parse_meta(os.path.join(out, "shard_%d.meta" % i))
+        want = audit_shards.parse_meta(os.path.join(LANE, "shard_%d.meta" % i))
+        for k in ("certs", "cert_sha256", "summary_line", "survivor_lines"):
+            assert got[k] == want[k], (i, k)
+    p = drive(str(tmp_path / "bad"), "--only", "7", bin_path=paths.ORACLE_BIN)
+    assert p.returncode == 1 and b"FATAL --only" in p.stdout
