@@ -222,6 +222,8 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
   tune_.table.clear();
   tune_.launch.graph = false;
   std::vector<std::pair<double, u64>> table;
+  struct Row { std::vector<Interval> wins; std::vector<std::pair<double, u64>> timed; };
+  std::vector<Row> rows;
   for (int lw = 16; lw <= 34; lw += 2) {
     const u64 W = 1ull << lw;
     const u64 n = std::max<u64>(1, std::min<u64>(256, (1ull << 30) / W));
@@ -229,6 +231,7 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     for (u64 i = 0; i < n; ++i) wins.push_back(Interval{base + i * W, base + (i + 1) * W});
     u64 best = 0;
     double best_ms = 1e300;
+    std::vector<std::pair<double, u64>> timed;
     for (const Estimate &e : est_) {
       tune_.fixed_Mp = e.Mp;
       double ms;
@@ -237,8 +240,10 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
                (unsigned long long)e.Mp, ms);
       report += line;
       if (ms < best_ms) { best_ms = ms; best = e.Mp; }
+      timed.emplace_back(ms, e.Mp);
     }
     table.emplace_back((double)lw, best);
+    rows.push_back(Row{wins, timed});
   }
   tune_.fixed_Mp = saved.fixed_Mp;
   tune_.table = table;
@@ -257,6 +262,34 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
         if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
       }
     tune_.launch = best_cfg;
+    // The layouts above were timed at the default launch shape, and close ones can rank
+    // the other way at the tuned shape (one 2^34 window at 10^15: wheel 2042040 is 14%
+    // ahead of 120120 at block 256, 37% at block 512 with 16384-position tiles). So every
+    // layout within 1.5x of its row's best is timed again at the tuned shape, and the
+    // fastest there takes the row.
+    for (size_t i = 0; i < rows.size(); ++i) {
+      const Row &row = rows[i];
+      double row_best = 1e300;
+      for (const auto &c : row.timed) row_best = std::min(row_best, c.first);
+      u64 best = table[i].second;
+      double best_ms = 1e300;
+      int n = 0;
+      for (const auto &c : row.timed) {
+        if (c.first > 1.5 * row_best) continue;
+        ++n;
+        tune_.fixed_Mp = c.second;
+        double ms;
+        if (time_batch(row.wins, ms)) { tune_ = saved; return -1; }
+        if (ms < best_ms) { best_ms = ms; best = c.second; }
+      }
+      tune_.fixed_Mp = saved.fixed_Mp;
+      if (n > 1) {
+        snprintf(line, sizeof line, "refine W=2^%.0f Mp=%llu sieve_ms=%.4f (%d layouts)\n", table[i].first,
+                 (unsigned long long)best, best_ms, n);
+        report += line;
+      }
+      tune_.table[i].second = best;
+    }
   }
   tune_.launch.graph = saved.launch.graph;
   for (const auto &e : tune_.table) {
