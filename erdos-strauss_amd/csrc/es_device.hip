@@ -168,12 +168,13 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
 // off_words count the map and the offset row in 32-bit words. The offset rows (one per
 // lane, megabytes in all) do not stay in L2, so each tile streams its row into LDS with
 // coalesced 16-byte loads that overlap the mask init, instead of every bin waiting on
-// its own global load.
+// its own global load. kRowInLds = false (with off_words = 0) is the form for a row
+// that does not fit beside the map: it is read from global memory.
 //
 // Each tile writes its counters {positions, alive after covers, survivors} to its own
 // slot of tile_counts: no counter is shared between workgroups (the host sums them per
 // window), so only tiles with survivors touch a shared word (the survivor count).
-template <bool kPersistent>
+template <bool kPersistent, bool kRowInLds>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
                                                 const LayoutDev *__restrict__ lays,
@@ -204,12 +205,12 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
     const u32 P = L->P;
-    {
-      const uint4 *src = (const uint4 *)(L->off + (size_t)td.lane * L->ostride);
+    const u16 *grow = L->off + (size_t)td.lane * L->ostride;
+    if (kRowInLds) {
+      const uint4 *src = (const uint4 *)grow;
       uint4 *dst = (uint4 *)soff;
       for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
     }
-    const u16 *offrow = soff;
 
     // ---- init: wheel mask and tail mask, expanded to one byte per position
     u32 my_pos = 0;
@@ -229,17 +230,25 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       }
     }
     __syncthreads();
-    // ---- stage B: certified filters
+    // ---- stage B: certified filters. The offset row is read from LDS where it was
+    // staged (kRowInLds), else from global memory.
     const bool wide = (td.k0 >> 32) != 0;
-    if (wide) mark_range<true>(map, nb, L, offrow, td.k0, 0, L->ncov, wave, nwaves, lane64);
-    else mark_range<false>(map, nb, L, offrow, td.k0, 0, L->ncov, wave, nwaves, lane64);
+    auto mark = [&](u32 from, u32 to) {
+      if (kRowInLds) {
+        if (wide) mark_range<true>(map, nb, L, soff, td.k0, from, to, wave, nwaves, lane64);
+        else mark_range<false>(map, nb, L, soff, td.k0, from, to, wave, nwaves, lane64);
+      } else {
+        if (wide) mark_range<true>(map, nb, L, grow, td.k0, from, to, wave, nwaves, lane64);
+        else mark_range<false>(map, nb, L, grow, td.k0, from, to, wave, nwaves, lane64);
+      }
+    };
+    mark(0, L->ncov);
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
       for (u32 c = tid; c < nc; c += bs) my_after += __popc(chunk_bits(map32, c));
       // ---- stage B': small primes
-      if (wide) mark_range<true>(map, nb, L, offrow, td.k0, L->ncov, L->nf, wave, nwaves, lane64);
-      else mark_range<false>(map, nb, L, offrow, td.k0, L->ncov, L->nf, wave, nwaves, lane64);
+      mark(L->ncov, L->nf);
       __syncthreads();
     }
     u32 my_surv = 0;
@@ -1178,7 +1187,7 @@ struct Slot {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_ok = false;
-  u32 graph_block = 0, graph_grid = 0;
+  u32 graph_block = 0, graph_grid = 0, graph_off_words = 0;
   size_t graph_lds = 0;
   bool graph_tail = false;
   TileDesc *d_tiles = nullptr;
@@ -1229,6 +1238,7 @@ struct DeviceCtx::Impl {
   int occupancy_blocks = 0;
   u32 occ_block = 0;
   size_t occ_lds = 0;
+  u32 occ_off_words = 0;
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
   Slot slot[DeviceCtx::kMaxInFlight];
   int head = 0;                     // the oldest batch in flight
@@ -1445,12 +1455,12 @@ int DeviceCtx::enqueue_batch(int si) {
     // not by separate event packets around it.
     const hipEvent_t e1 = timed ? b.ev1 : nullptr, e2 = timed ? b.ev2 : nullptr;
     if (cfg.persistent) {
-      hipExtLaunchKernelGGL(dev::k_sieve<true>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
+      hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<true, true> : dev::k_sieve<true, false>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
                             (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
     } else {
-      hipExtLaunchKernelGGL(dev::k_sieve<false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
+      hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<false, true> : dev::k_sieve<false, false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
                             (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
@@ -1478,7 +1488,7 @@ int DeviceCtx::enqueue_batch(int si) {
   HIP_TRY(hipEventRecord(b.ev0, b.stream));
   if (cfg.graph && cfg.persistent) {
     if (!b.graph_ok || b.graph_block != cfg.block || b.graph_lds != b.lds || b.graph_tail != cfg.device_tail ||
-        b.graph_grid != b.grid) {
+        b.graph_grid != b.grid || b.graph_off_words != b.off_words) {
       if (b.graph_exec) { HIP_TRY(hipGraphExecDestroy(b.graph_exec)); b.graph_exec = nullptr; }
       if (b.graph) { HIP_TRY(hipGraphDestroy(b.graph)); b.graph = nullptr; }
       HIP_TRY(hipStreamBeginCapture(m.capture, hipStreamCaptureModeThreadLocal));
@@ -1494,6 +1504,7 @@ int DeviceCtx::enqueue_batch(int si) {
       b.graph_lds = b.lds;
       b.graph_tail = cfg.device_tail;
       b.graph_grid = b.grid;
+      b.graph_off_words = b.off_words;
     }
     HIP_TRY(hipGraphLaunch(b.graph_exec, b.stream));
   } else if (enqueue(b.stream, true)) {
@@ -1554,26 +1565,33 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
     const u32 lds_bytes = (tile_max + 31u) & ~31u;
     b.lds_words = lds_bytes / 4;
+    // The tile's row of filter offsets is staged in LDS beside the map when both fit.
+    // Large generated filter sets (about 77,000 marking units at mmax 10000, a 154 KB
+    // row) leave no room for a long tile beside it: then k_sieve reads the row from
+    // global memory instead (off_words = 0).
+    const u32 scalar_bytes = (16 + 3 * dev::kMaxWaves) * sizeof(u32);
     b.off_words = m.off_max / 2;
-    b.lds = lds_bytes + b.off_words * 4 + (16 + 3 * dev::kMaxWaves) * sizeof(u32);
+    if (lds_bytes + b.off_words * 4 + scalar_bytes > lds_limit_) b.off_words = 0;
+    b.lds = lds_bytes + b.off_words * 4 + scalar_bytes;
     if (b.lds > lds_limit_) {
       err_ = "tile of " + std::to_string(tile_max) + " positions and " + std::to_string(m.off_max) +
              " filter offsets need " + std::to_string(b.lds) + " B of LDS, above the device's " +
              std::to_string(lds_limit_);
       return -1;
     }
-    if (m.occ_block != cfg.block || m.occ_lds != b.lds) {
+    if (m.occ_block != cfg.block || m.occ_lds != b.lds || m.occ_off_words != b.off_words) {
       if (b.lds > 65536) {
-        HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)b.lds));
-        HIP_TRY(hipFuncSetAttribute((const void *)dev::k_sieve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)b.lds));
+        for (const void *f : {(const void *)dev::k_sieve<true, true>, (const void *)dev::k_sieve<true, false>,
+                              (const void *)dev::k_sieve<false, true>, (const void *)dev::k_sieve<false, false>})
+          HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
       }
       int nb = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true>, cfg.block, b.lds));
+      if (b.off_words) HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true, true>, cfg.block, b.lds));
+      else HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dev::k_sieve<true, false>, cfg.block, b.lds));
       m.occupancy_blocks = std::max(1, nb);
       m.occ_block = cfg.block;
       m.occ_lds = b.lds;
+      m.occ_off_words = b.off_words;
     }
     b.grid = (u32)((size_t)cus_ * m.occupancy_blocks);
     if (enqueue_batch(si)) return -1;

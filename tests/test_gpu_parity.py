@@ -135,6 +135,10 @@ SURV_CASES = [
     ("w204_1e12", 10 ** 12, 10 ** 12 + 10 ** 9, dict(filters=F204)),
     ("w204_top", TOP - 2 ** 31, TOP, dict(filters=F204)),
     ("mmax200", 10 ** 12, 10 ** 12 + 10 ** 8, dict(mmax=200)),
+    # The command line's default filter set (covers generated to mmax 10000, ~77,000
+    # marking units): a 154 KB offset row, staged in LDS beside short tiles (wheel
+    # 2042040) and read from global memory beside long ones (the small sub-wheels).
+    ("generated", 10 ** 12, 10 ** 12 + 10 ** 8, dict()),
     ("sieve", 10 ** 12, 10 ** 12 + 10 ** 9, dict(filters=F204, sieve=1000)),
     ("sieve_w120", 10 ** 11, 10 ** 11 + 10 ** 8, dict(wheel=120120, filters=F120, sieve=3000)),
 ]
