@@ -116,6 +116,18 @@ def test_record_via_cli(name):
     assert out == paths.golden_record(name)
 
 
+def test_cli_default_filters():
+    """The binary with no flags at all: covers generated to mmax 10000 (the reference's
+    default), whose offset rows are too long to stage beside long tiles. Same stdout
+    and exit code as the oracle's record."""
+    lo, hi = 10 ** 9, 10 ** 9 + 10 ** 10      # 11.3 M positions, 13,223 composite survivors
+    print("cli default", lo, hi, flush=True)
+    want, want_rc = oracle_record(lo, hi, dict(wheel=2042040, mmax=10000, sieve=0, filters=None))
+    got_rc, out, err = paths.run(paths.SIEVE_BIN, [str(lo), str(hi)], timeout=300)
+    assert got_rc == want_rc, err.decode()
+    assert out.decode() == want
+
+
 @pytest.mark.parametrize("k", [6, 9, 13, 16, 19])
 def test_decade_records(k):
     """The reference's own records data/rederive/decades/d<k>.txt."""
