@@ -247,6 +247,9 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     u32 my_after = 0;
     if (sieve) {
       for (u32 c = tid; c < nc; c += bs) my_after += __popc(chunk_bits(map32, c));
+      // Every wave's count of the covers-only map completes before any wave's stage B'
+      // store: a wave's chunks are not the bytes it marks.
+      __syncthreads();
       // ---- stage B': small primes
       mark(L->ncov, L->nf);
       __syncthreads();
