@@ -182,6 +182,16 @@ int es_primality(es_scanner *s, const uint64_t *vals, size_t n, uint8_t *flags) 
   return ES_OK;
 }
 
+static_assert(ES_NOALLOC_MAX_VALUES == es::kNoAllocMaxValues, "header and device agree");
+
+int es_primality_noalloc(es_scanner *s, const uint64_t *d_vals, size_t n, uint8_t *d_flags, void *stream) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  const int rc = s->sc.device().primality_noalloc(d_vals, n, d_flags, stream);
+  if (rc == -2) return fail(ES_ERR_ARG, s->sc.device().last_error());
+  if (rc) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  return ES_OK;
+}
+
 int es_certify(uint64_t p, char *buf, size_t cap) {
   std::string line;
   const int kind = es::certify_prime(p, line);

@@ -51,6 +51,10 @@ struct BatchResult {
 // default (the rest of the kernel's LDS is a few hundred bytes).
 constexpr u32 kMaxTileBits = 61440;
 
+// Values one call of the allocation-free primality entry point may test: its scratch
+// (a count and one u64 per value) is a fixed device array of 8 MiB per GPU.
+constexpr size_t kNoAllocMaxValues = size_t(1) << 20;
+
 struct LaunchCfg {
   bool persistent = true;   // work-queue persistent kernel vs one workgroup per tile
   bool graph = false;       // replay a captured hipGraph (persistent form only)
@@ -83,6 +87,10 @@ class DeviceCtx {
   static constexpr int kMaxInFlight = 4;
   // Deterministic MR of arbitrary values on the device (parity checks).
   int primality(const u64 *vals, size_t n, uint8_t *flags);
+  // The same test on caller-owned device buffers, enqueued on `stream` without waiting and
+  // without allocating (state in the library's fixed device scratch). -2 for an argument
+  // error (too many values, not device memory), -1 for a HIP failure.
+  int primality_noalloc(const u64 *d_vals, size_t n, uint8_t *d_flags, void *stream);
   // Stage D on the device for primes vals[0..n) (one RungResult each).
   int rung(const u64 *vals, size_t n, RungResult *out);
   // Measured LDS atomic-AND rate of the whole device (32-bit operations per second).
@@ -108,6 +116,7 @@ class DeviceCtx {
   int cus_ = 0;
   size_t lds_limit_ = 0;   // dynamic LDS one workgroup may allocate (bytes)
   size_t layout_bytes_ = 0;
+  void *scratch_ = nullptr;   // this device's copy of the scratch array (device address)
 };
 
 }  // namespace es

@@ -32,6 +32,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -473,6 +475,23 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
     }
     if (!ok) flags[i] = kComposite;
   }
+}
+
+// Scratch of the allocation-free primality entry point, one copy per GPU for the life of
+// the process: [0] the value count (u32), [kScratchOne ..) 2^64 mod n per value, written
+// by k_mr_base2 before k_mr_bases reads it. Host code reaches it only through the device
+// address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchOne = 32;   // 256 bytes in: the per-value words stay 256-aligned
+static __device__ __align__(256) u64 scratch[kScratchOne + kNoAllocMaxValues];
+
+__global__ void k_set_count(u32 *count, u32 n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
+}
+
+// Verdicts to the public form: 1 = prime (kPrime, or kPending after every base), 0 = not.
+__global__ void k_flags_to_prime(u8 *__restrict__ flags, u32 n) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flags[i] = flags[i] != kComposite ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ stage D on the device
@@ -1248,6 +1267,13 @@ struct DeviceCtx::Impl {
   int busy = 0;                     // batches in flight (0 .. kMaxInFlight)
 };
 
+namespace {
+// Device address of dev::scratch per device ordinal, looked up once per device with that
+// device current (each GPU holds its own copy of the array at its own address).
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_addr;
+}  // namespace
+
 DeviceCtx::DeviceCtx() : impl_(new Impl) {}
 
 DeviceCtx::~DeviceCtx() {
@@ -1294,6 +1320,17 @@ int DeviceCtx::open(int device) {
   if (device < 0 || device >= n) { err_ = "no such HIP device"; return -1; }
   HIP_TRY(hipSetDevice(device));
   device_ = device;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it == g_scratch_addr.end()) {
+      void *ptr = nullptr;
+      HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(dev::scratch)));
+      if (!ptr) { err_ = "hipGetSymbolAddress: null address for the scratch array"; return -1; }
+      it = g_scratch_addr.emplace(device, ptr).first;
+    }
+    scratch_ = it->second;
+  }
   hipDeviceProp_t p;
   HIP_TRY(hipGetDeviceProperties(&p, device));
   cus_ = p.multiProcessorCount;
@@ -1784,6 +1821,43 @@ int DeviceCtx::primality(const u64 *vals, size_t n, uint8_t *flags) {
   HIP_TRY(hipStreamSynchronize(m.stream));
   for (size_t i = 0; i < n; ++i) flags[i] = flags[i] != 0;
   for (void *p : {(void *)d_v, (void *)d_one, (void *)d_f, (void *)d_n}) HIP_TRY(hipFree(p));
+  return 0;
+}
+
+// The kernels of primality() on caller-owned device buffers, with the count and the
+// per-value Montgomery ones in this device's copy of dev::scratch: nothing is allocated
+// and nothing waits. Pointers are checked to be device memory first, so a host pointer is
+// an error here and not a fault in a kernel.
+int DeviceCtx::primality_noalloc(const u64 *d_vals, size_t n, uint8_t *d_flags, void *stream) {
+  if (device_ < 0 || !scratch_) { err_ = "device not open"; return -1; }
+  if (n > kNoAllocMaxValues) {
+    err_ = "es_primality_noalloc: " + std::to_string(n) + " values exceed the scratch maximum of " +
+           std::to_string(kNoAllocMaxValues);
+    return -2;
+  }
+  if (n == 0) return 0;
+  if (!d_vals || !d_flags) { err_ = "es_primality_noalloc: null buffer"; return -2; }
+  HIP_TRY(hipSetDevice(device_));
+  for (const void *ptr : {(const void *)d_vals, (const void *)d_flags}) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess || a.type != hipMemoryTypeDevice || a.device != device_) {
+      (void)hipGetLastError();
+      err_ = "es_primality_noalloc: buffers must be device memory of the scanner's device";
+      return -2;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  u32 *count = (u32 *)scratch_;
+  u64 *one = (u64 *)scratch_ + dev::kScratchOne;
+  const u32 n32 = (u32)n;
+  dev::k_set_count<<<1, 64, 0, st>>>(count, n32);
+  HIP_TRY(hipGetLastError());
+  dev::k_mr_base2<<<std::max(1, cus_ * 4), 256, 0, st>>>(d_vals, count, n32, d_flags, one);
+  HIP_TRY(hipGetLastError());
+  dev::k_mr_bases<<<std::max(1, cus_ * 8), 256, 0, st>>>(d_vals, count, n32, d_flags, one);
+  HIP_TRY(hipGetLastError());
+  dev::k_flags_to_prime<<<std::max(1, cus_), 256, 0, st>>>(d_flags, n32);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 

@@ -98,6 +98,8 @@ def lib():
             "es_survivors": (ctypes.c_int, [ctypes.c_void_p, U64, U64, P(U64), P(ctypes.c_uint8), ctypes.c_size_t,
                                             P(ctypes.c_size_t), P(EsCounts)]),
             "es_primality": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_uint8)]),
+            "es_primality_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                    ctypes.c_void_p, ctypes.c_void_p]),
             "es_certify": (ctypes.c_int, [U64, ctypes.c_char_p, ctypes.c_size_t]),
             "es_set_layout": (ctypes.c_int, [ctypes.c_void_p, U64]),
             "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),

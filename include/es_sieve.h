@@ -113,6 +113,20 @@ int es_survivors(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t *p, uint8_t *
  * --check-primality (sieve/cuda/sieve_cuda.cu:89-102). */
 int es_primality(es_scanner *s, const uint64_t *vals, size_t n, uint8_t *flags);
 
+/* es_primality for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It has no counterpart in the
+ * reference. d_vals (n values) and d_flags (n verdicts out, 1 iff prime, the same bytes
+ * es_primality writes) are device memory of the scanner's device, allocated by the
+ * caller; the kernels are enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting. Nothing is allocated: the count and a 64-bit word
+ * per value live in a fixed device array of the library, one copy per GPU held until the
+ * process ends, so n may not exceed ES_NOALLOC_MAX_VALUES (ES_ERR_ARG beyond it, or for a
+ * pointer that is not device memory). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race; calls in order on one stream
+ * do not. */
+#define ES_NOALLOC_MAX_VALUES (1u << 20)
+int es_primality_noalloc(es_scanner *s, const uint64_t *d_vals, size_t n, uint8_t *d_flags, void *stream);
+
 /* Stage D then stage E for one prime p, on the host: writes "RUNG p A r u\n",
  * "SOLVED p a c d k\n" or "SURVIVOR p\n" into buf and returns 0, 1 or 2 (negative if
  * buf is too small). Replaces ref3_solve_rung / ref3_solve_direct
