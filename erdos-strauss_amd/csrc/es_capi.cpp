@@ -75,7 +75,7 @@ static int run_windows(es_scanner *s, const uint64_t *lo, const uint64_t *hi, si
   std::string msg;
   for (size_t i = 0; i < nwin; ++i) {
     w[i] = es::Interval{lo[i], hi[i]};
-    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg))
+    if (es::check_interval(s->sc.plan(), lo[i], hi[i], msg))
       return fail(ES_ERR_CONFIG, msg);
   }
   if (s->sc.scan(w, recs, s->last, msg)) return fail(ES_ERR_DEVICE, msg);
@@ -121,7 +121,7 @@ int es_scan_submit(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t
   std::string msg;
   for (size_t i = 0; i < nwin; ++i) {
     w[i] = es::Interval{lo[i], hi[i]};
-    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
+    if (es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
   }
   if (s->sc.submit(w, msg)) return fail(ES_ERR_DEVICE, msg);
   return ES_OK;
@@ -162,7 +162,7 @@ int es_survivors(es_scanner *s, uint64_t lo, uint64_t hi, uint64_t *p, uint8_t *
   std::vector<uint8_t> fl;
   std::vector<es::WindowRecord> recs;
   std::string msg;
-  if (hi > lo && es::check_interval(s->sc.plan(), lo, hi, msg)) return fail(ES_ERR_CONFIG, msg);
+  if (es::check_interval(s->sc.plan(), lo, hi, msg)) return fail(ES_ERR_CONFIG, msg);
   if (s->sc.survivors(w, recs, sp, fl, s->last, msg)) return fail(ES_ERR_DEVICE, msg);
   *n = sp.size();
   for (size_t i = 0; i < sp.size() && i < cap; ++i) {
@@ -275,7 +275,7 @@ int es_autotune(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nw
   std::string msg, rep;
   for (size_t i = 0; i < nwin; ++i) {
     w[i] = es::Interval{lo[i], hi[i]};
-    if (hi[i] > lo[i] && es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
+    if (es::check_interval(s->sc.plan(), lo[i], hi[i], msg)) return fail(ES_ERR_CONFIG, msg);
   }
   if (s->sc.autotune(w, rep, msg)) return fail(ES_ERR_DEVICE, msg);
   if (report) {

@@ -67,7 +67,7 @@ u64 Server::submit(u64 lo, u64 hi) {
   r.info.arrival_ms = now_ms();
   ++st_.submitted;
   std::string msg;
-  if (hi > lo && check_interval(sc_.plan(), lo, hi, msg)) {
+  if (check_interval(sc_.plan(), lo, hi, msg)) {
     // rung_scan3 refuses before scanning (rung_scan3.cpp:1057-1069): exit 1, no record.
     r.record = msg;
     r.info.exit_code = 1;

@@ -78,6 +78,8 @@ def test_cli_refusals():
     """Exit code 1 and nothing on stdout for the reference's refusals."""
     F = os.path.join(paths.FILTERS, "f_M2042040_mmax2000_gpu.txt")
     for argv in (["1000", "2000"],                                      # LO below the pmin floor
+                 ["1000", "1000"],                                      # the same, empty
+                 ["18446744065119614977", "18446744065119614977", "--filters", F],   # HI above, empty
                  ["1000000000000", "18446744065119614977", "--filters", F],   # HI above the ceiling
                  ["1000000000000", "1000000001000", "--bogus"],          # unknown option
                  ["1000000000000", "1000000001000", "--filters", "/nonexistent"]):
@@ -88,3 +90,58 @@ def test_cli_refusals():
 def test_cli_check_factor():
     rc, out, _ = paths.run(paths.SIEVE_BIN, ["--check-factor", "40"])
     assert rc == 0 and out.startswith(b"ok")
+
+
+BANNER_CASES = [
+    ("gpu_table", ["--filters", "erdos-strauss_amd/tables/filters/f_M2042040_mmax2000_gpu.txt"]),
+    ("w120_runtime", ["--wheel", "120120", "--filters", "erdos-strauss_amd/tables/filters/f_M120120_mmax2000_runtime.txt"]),
+    ("mi355x_table", ["--filters", "erdos-strauss_amd/tables/filters/f_M2042040_mmax2000_mi355x.txt"]),
+    ("generated_mmax2000", ["--mmax", "2000"]),
+    ("generated_w120_mmax200", ["--wheel", "120120", "--mmax", "200"]),
+    ("sieve1000", ["--filters", "erdos-strauss_amd/tables/filters/f_M2042040_mmax2000_gpu.txt", "--sieve", "1000"]),
+]
+
+
+def _digest_line(err, prefix):
+    lines = [l for l in err.decode().splitlines() if l.startswith(prefix + " filters=")]
+    assert len(lines) == 1, err
+    return lines[0][len(prefix):]
+
+
+@pytest.mark.skipif(not os.path.exists(paths.REF_BIN), reason="reference scanner not built from source")
+@pytest.mark.parametrize("name,flags", BANNER_CASES, ids=[c[0] for c in BANNER_CASES])
+def test_filter_digest_banner_matches_reference(name, flags):
+    """The stderr banner's filter line -- count, FNV-1a digest over "mod,res;" of the
+    certified filters (fnv64=), source and sieve bound (rung_scan3.cpp:1113-1126) -- is
+    the reference scanner's, for table files, generated covers and --sieve. An empty
+    interval needs no device, so this runs anywhere."""
+    argv = ["10000000000000", "10000000000000"] + flags
+    rc, out, err = paths.run(paths.SIEVE_BIN, argv)
+    rrc, rout, rerr = paths.run(paths.REF_BIN, argv)
+    assert rc == rrc == 0
+    assert out == rout                                   # the empty record
+    assert _digest_line(err, "[es_sieve]") == _digest_line(rerr, "[rung_scan3]")
+
+
+def test_capi_refuses_empty_windows_outside_the_range():
+    """Through the C ABI, an empty window is refused exactly as the CLI and rung_scan3
+    refuse it (rung_scan3.cpp:1057-1069 run whether or not the interval is empty): LO at
+    or below the pmin floor, or HI above the stage-E ceiling. A scanner opened for
+    host-side planning only (device -1) makes the check, so this runs without a GPU."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("es_sieve", os.path.join(paths.PKG, "es_sieve.py"))
+    es = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(es)
+    top = 18446744065119614976
+    with es.Scanner(filters=es.F204, device=-1) as sc:
+        for lo, hi in [(1000, 1000), (255254, 255254), (1000, 2000), (top + 1, top + 1), (10 ** 13, top + 1)]:
+            assert not sc.check_interval(lo, hi)
+            with pytest.raises(es.EsError, match="FATAL: (LO|HI) must"):
+                sc.records([(lo, hi)])
+            with pytest.raises(es.EsError, match="FATAL: (LO|HI) must"):
+                sc.submit([(10 ** 13, 10 ** 13 + 1), (lo, hi)])
+        assert sc.in_flight() == 0
+        # A valid empty window passes the check; scanning it then needs the device.
+        assert sc.check_interval(10 ** 13, 10 ** 13) and sc.check_interval(top, top)
+        with pytest.raises(es.EsError, match="device"):
+            sc.records([(10 ** 13, 10 ** 13)])

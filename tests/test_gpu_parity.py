@@ -199,12 +199,15 @@ def test_launch_shapes(persistent, block, bits):
 
 def test_graph_replays_match_eager():
     """The captured hipGraph of reset + sieve + MR, replayed over different batches with
-    eager work in between, gives the eager records every time."""
+    eager work in between, gives the eager records every time, and those are the
+    oracle's."""
     kw = dict(wheel=2042040, mmax=10000, sieve=0, filters=F204)
     sc = scanner(**kw)
     batches = [[(10 ** 13, 10 ** 13 + 10 ** 9)], [(10 ** 15, 10 ** 15 + 2 ** 30), (10 ** 16, 10 ** 16 + 2 ** 24)],
                [(TOP - 2 ** 28, TOP)], [(10 ** 13, 10 ** 13 + 10 ** 9)]]
     eager = [sc.records(b)[0] for b in batches]
+    for b, recs in zip(batches, eager):                 # the eager records are the oracle's
+        assert recs == [oracle_record(lo, hi, kw)[0] for lo, hi in b]
     sc.set_graph(True)
     try:
         for rep in range(3):
@@ -223,14 +226,16 @@ def test_pipelined_batches_match_records(graph, depth):
     """submit/collect with up to `depth` batches in flight (the host tail of one batch
     overlapping the device work of the next ones, each slot on its own stream) returns,
     batch by batch and in submission order, the records records() gives one batch at a
-    time: uneven batches, a batch with an empty window, an empty batch, each slot reused
-    several times, graph on and off."""
+    time (and the oracle gives): uneven batches, a batch with an empty window, an empty
+    batch, each slot reused several times, graph on and off."""
     kw = dict(wheel=2042040, mmax=10000, sieve=0, filters=F204)
     sc = scanner(**kw)
     batches = [[(10 ** 15, 10 ** 15 + 2 ** 30)], [(10 ** 13, 10 ** 13 + 10 ** 9), (10 ** 16, 10 ** 16)],
                [(TOP - 2 ** 28, TOP)], [], [(10 ** 16, 10 ** 16 + 2 ** 24), (10 ** 12, 10 ** 12 + 2 ** 20)],
                [(10 ** 15, 10 ** 15 + 2 ** 30)]]
     want = [sc.records(b) for b in batches]
+    for b, (recs, codes) in zip(batches, want):         # what they are compared with is the oracle's
+        assert list(zip(recs, codes)) == [oracle_record(lo, hi, kw) for lo, hi in b]
     sc.set_graph(graph)
     try:
         for rep in range(2):
