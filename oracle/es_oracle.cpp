@@ -24,6 +24,8 @@
 //                      survivor in lane order instead of the record (kernel parity)
 //   --solve P...       print the stage D/E line for each given prime (golden lines)
 //   --solve-direct P.. print the stage E line alone (the reference's --check-direct rows)
+//   --batch FILE       one record per "lo hi" line of FILE, tables built once (the
+//                      serving trace's golden digests)
 #include <algorithm>
 #include <cinttypes>
 #include <cmath>
@@ -589,7 +591,10 @@ u64 small_primes_append(u64 P, u64 M, vector<u64> &mods, vector<u64> &ress) {
   return n;
 }
 
-int scan(Options o) {
+// One record per interval. Without a batch: the interval in o (after --shard), exactly as
+// rung_scan3 refuses and scans it. With a batch (--batch FILE): every "lo hi" line of the
+// file in turn, with the tables built once; a refused interval's record is its FATAL line.
+int scan(Options o, const vector<std::pair<u64, u64>> *batch) {
   if (!o.seg) o.seg = 262144;
   if (o.nshard == 0) o.nshard = 1;
   if (o.nshard > 1) {                                   // rung_scan3.cpp:1022-1027
@@ -610,15 +615,20 @@ int scan(Options o) {
   } else {
     fprintf(stderr, "[es_oracle] WARNING: no pinned digest for M=%" PRIu64 " -- running UNPINNED\n", o.M);
   }
-  if (o.hi > kDirectPMax) {
-    fprintf(stderr, "FATAL: HI must not exceed %" PRIu64 ", the stage-E ceiling\n", kDirectPMax);
-    return 1;
-  }
   const u64 floor_ = std::max(w.max_pmin, cover_pmax);
-  if (o.lo <= floor_) {
-    fprintf(stderr, "FATAL: LO must exceed max pmin of the filter table (%" PRIu64 ")\n", floor_);
-    return 1;
-  }
+  // rung_scan3.cpp:1057-1069; in a batch the message goes to stdout as the record.
+  auto refused = [&](u64 lo, u64 hi, FILE *to) {
+    if (hi > kDirectPMax) {
+      fprintf(to, "FATAL: HI must not exceed %" PRIu64 ", the stage-E ceiling\n", kDirectPMax);
+      return true;
+    }
+    if (lo <= floor_) {
+      fprintf(to, "FATAL: LO must exceed max pmin of the filter table (%" PRIu64 ")\n", floor_);
+      return true;
+    }
+    return false;
+  };
+  if (!batch && refused(o.lo, o.hi, stderr)) return 1;
   vector<u64> mods, ress;
   if (o.filters) {
     FILE *fp = fopen(o.filters, "r");
@@ -643,28 +653,40 @@ int scan(Options o) {
   fprintf(stderr, "[es_oracle] wheel M=%" PRIu64 " lanes=%zu filters=%zu (+%zu primes)\n", o.M,
           w.classes.size(), ncov, mods.size() - ncov);
   init_trial();                                         // before the threads start
-  const size_t L = w.classes.size();
-  vector<Stats> lst(L);
-  vector<vector<string>> lout(L);
 #ifdef _OPENMP
   if (o.threads > 0) omp_set_num_threads(o.threads);
+#endif
+  const size_t L = w.classes.size();
+  auto one = [&](const Options &oi) -> int {
+    vector<Stats> lst(L);
+    vector<vector<string>> lout(L);
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
-  for (size_t i = 0; i < L; ++i) sweep(w.classes[i], o, mods, ress, plan, ncov, lst[i], lout[i]);
-  Stats st;
-  for (size_t i = 0; i < L; ++i) {
-    st.add(lst[i]);
-    for (const string &s : lout[i])
-      if (o.emit_surv || s.compare(0, 9, "SURVIVOR ") != 0) fputs(s.c_str(), stdout);
+    for (size_t i = 0; i < L; ++i) sweep(w.classes[i], oi, mods, ress, plan, ncov, lst[i], lout[i]);
+    Stats st;
+    for (size_t i = 0; i < L; ++i) {
+      st.add(lst[i]);
+      for (const string &s : lout[i])
+        if (oi.emit_surv || s.compare(0, 9, "SURVIVOR ") != 0) fputs(s.c_str(), stdout);
+    }
+    if (oi.dump_surv) return 0;
+    printf("SUMMARY lo=%" PRIu64 " hi=%" PRIu64 " wheelM=%" PRIu64 " lanes=%zu mmax=%" PRIu64
+           " ucert_rmax=%u ucert_umax=%u wheel_pmin=%" PRIu64 " classes_sha256=%s positions=%" PRIu64
+           " covered=%" PRIu64 " sieved=%" PRIu64 " mr=%" PRIu64 " composite=%" PRIu64 " rung=%" PRIu64
+           " direct=%" PRIu64 " survivors=%" PRIu64 "\n",
+           oi.lo, oi.hi, oi.M, L, oi.mmax, kUcertR, kUcertU, w.max_pmin, pin ? pin->sha : "unpinned",
+           st.positions, st.covered, st.sieved, st.mr, st.composite, st.rung, st.direct, st.survivors);
+    return st.survivors ? 2 : 0;
+  };
+  if (!batch) return one(o);
+  for (const auto &iv : *batch) {
+    Options oi = o;
+    oi.lo = iv.first;
+    oi.hi = iv.second;
+    if (!refused(oi.lo, oi.hi, stdout)) one(oi);
   }
-  if (o.dump_surv) return 0;
-  printf("SUMMARY lo=%" PRIu64 " hi=%" PRIu64 " wheelM=%" PRIu64 " lanes=%zu mmax=%" PRIu64
-         " ucert_rmax=%u ucert_umax=%u wheel_pmin=%" PRIu64 " classes_sha256=%s positions=%" PRIu64
-         " covered=%" PRIu64 " sieved=%" PRIu64 " mr=%" PRIu64 " composite=%" PRIu64 " rung=%" PRIu64
-         " direct=%" PRIu64 " survivors=%" PRIu64 "\n",
-         o.lo, o.hi, o.M, L, o.mmax, kUcertR, kUcertU, w.max_pmin, pin ? pin->sha : "unpinned",
-         st.positions, st.covered, st.sieved, st.mr, st.composite, st.rung, st.direct, st.survivors);
-  return st.survivors ? 2 : 0;
+  return 0;
 }
 
 }  // namespace
@@ -719,16 +741,30 @@ int main(int argc, char **argv) {
     }
     return 0;
   }
+  vector<std::pair<u64, u64>> batch;
+  const bool batch_mode = argc >= 3 && !strcmp(argv[1], "--batch");
+  if (batch_mode) {
+    FILE *fp = fopen(argv[2], "r");
+    if (!fp) { fprintf(stderr, "FATAL: cannot open %s\n", argv[2]); return 1; }
+    unsigned long long a, b;
+    char line[256];
+    while (fgets(line, sizeof line, fp))
+      if (sscanf(line, "%llu %llu", &a, &b) == 2) batch.emplace_back(a, b);
+    fclose(fp);
+  }
   if (argc < 3) {
     fprintf(stderr, "usage: %s LO HI [--mmax N] [--wheel M] [--seg N] [--threads N] [--shard i/n]\n"
                     "          [--progress N] [--filters FILE] [--sieve P] [--no-emit-survivors]\n"
                     "          [--dump-survivors]\n"
-                    "       %s --verify | --dump-classes [--wheel M] | --solve P...\n", argv[0], argv[0]);
+                    "       %s --batch FILE [the same options]\n"
+                    "       %s --verify | --dump-classes [--wheel M] | --solve P...\n", argv[0], argv[0], argv[0]);
     return 1;
   }
   Options o;
-  o.lo = strtoull(argv[1], 0, 10);
-  o.hi = strtoull(argv[2], 0, 10);
+  if (!batch_mode) {
+    o.lo = strtoull(argv[1], 0, 10);
+    o.hi = strtoull(argv[2], 0, 10);
+  }
   for (int i = 3; i < argc; ++i) {
     const char *a = argv[i];
     const bool has = i + 1 < argc;
@@ -746,6 +782,6 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "--dump-survivors")) o.dump_surv = true;
     else { fprintf(stderr, "FATAL: unknown option '%s'\n", a); return 1; }
   }
-  return scan(o);
+  return scan(o, batch_mode ? &batch : nullptr);
 }
 #endif

@@ -18,10 +18,11 @@ What it writes:
   c5_trace.txt           Poisson request trace, 200 req/s, numpy PCG64 seed 1
   c4/records_w<M>.txt    per-window records of the 512 batched windows (--c4-only)
   c5/records_first400.txt  records of the trace's first 400 requests (--c5-only)
+  c5/records_sha256.txt  sha256 of every request's record over the whole trace (--c5-digests)
   shards/lane/shard_*.meta shard metas of a 7-shard run of the reference binary (--shards-only)
 
-Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only | --shards-only |
-                                            --only NAME[,NAME...]]
+Usage: python3 tests/golden/make_golden.py [--skip-slow | --c4-only | --c5-only | --c5-digests |
+                                            --shards-only | --only NAME[,NAME...]]
   --skip-slow  leave out the top-of-range interval, whose reference startup
                (a base-prime sieve to sqrt(hi)) takes minutes and ~9 GB.
 """
@@ -224,6 +225,53 @@ def c5_records():
     print("c5 records", len(out), "cross-checked against the reference:", checked, flush=True)
 
 
+def split_records(data):
+    out, cur = [], []
+    for line in data.splitlines(keepends=True):
+        cur.append(line)
+        if line.startswith(b"SUMMARY ") or line.startswith(b"FATAL"):
+            out.append(b"".join(cur))
+            cur = []
+    return out
+
+
+def c5_digests(ref_every=8):
+    """sha256 of the record of every request of the serving trace (wheel 2042040, the GPU
+    filter table), from the oracle in batch mode. The first C5_GOLDEN must equal the
+    committed records, and every ref_every-th request with hi < 1e15 (where the reference
+    binary's startup is cheap) is also run through the reference binary and must agree
+    byte for byte."""
+    rows = [l.split() for l in open(os.path.join(HERE, "c5_trace.txt")) if not l.startswith("#")]
+    tmp = os.path.join(REPO, "_work", "c5_batch.txt")
+    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    with open(tmp, "w") as f:
+        for _, lo, hi in rows:
+            f.write("%s %s\n" % (lo, hi))
+    data = subprocess.run([ORACLE_BIN, "--batch", tmp, "--wheel", "2042040", "--filters", F204, "--threads", "8"],
+                          stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, check=True).stdout
+    recs = split_records(data)
+    if len(recs) != len(rows):
+        raise SystemExit("oracle batch gave %d records for %d requests" % (len(recs), len(rows)))
+    first = open(os.path.join(HERE, "c5", "records_first%d.txt" % C5_GOLDEN), "rb").read()
+    if b"".join(recs[:C5_GOLDEN]) != first:
+        raise SystemExit("oracle batch disagrees with the committed first records")
+    checked = 0
+    cheap = [i for i, (_, lo, hi) in enumerate(rows) if int(hi) < 10 ** 15]
+    for i in cheap[::ref_every]:
+        _, lo, hi = rows[i]
+        ref = subprocess.run([REF_BIN, lo, hi, "--wheel", "2042040", "--filters", F204], stdout=subprocess.PIPE,
+                             stderr=subprocess.DEVNULL).stdout
+        if ref != recs[i]:
+            raise SystemExit("oracle and reference disagree on request %d: %s %s" % (i, lo, hi))
+        checked += 1
+    with open(os.path.join(HERE, "c5", "records_sha256.txt"), "w") as f:
+        f.write("# sha256(record) lo hi per request of c5_trace.txt, in trace order (oracle, batch mode; "
+                "%d requests cross-checked against the reference binary)\n" % checked)
+        for (_, lo, hi), r in zip(rows, recs):
+            f.write("%s %s %s\n" % (hashlib.sha256(r).hexdigest(), lo, hi))
+    print("c5 digests", len(recs), "cross-checked against the reference:", checked, flush=True)
+
+
 SHARD_LO, SHARD_HI, SHARD_N = 10 ** 12 + 7, 10 ** 12 + 4 * 10 ** 10 + 3, 7
 
 
@@ -252,6 +300,9 @@ def main():
         return
     if "--c5-only" in sys.argv:
         c5_records()
+        return
+    if "--c5-digests" in sys.argv:
+        c5_digests()
         return
     if "--only" in sys.argv:
         names = set(sys.argv[sys.argv.index("--only") + 1].split(","))
