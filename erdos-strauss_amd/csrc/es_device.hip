@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "es_device.h"
+#include "es_montgomery.h"
 
 namespace es {
 namespace dev {
@@ -311,26 +312,23 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
 // serves every batch that fits the buffers; the host keeps ntiles within the buffers'
 // capacity. Every other per-batch output is written in full by the kernels.
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
-                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars) {
+                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars,
+                        u64 *__restrict__ board_words, u32 nboard_words) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const BatchHdr h = *h_hdr;
   if (g == 0) *hdr = h;
-  if (g < 4) scalars[g] = 0;   // survivor count, tile queue, stage-D queue
+  if (g < 4) scalars[g] = 0;   // survivor count, tile queue, stage-D queue, survivors finished
+  for (u32 i = g; i < nboard_words; i += stride) board_words[i] = 0;   // stage D's board of posted splits
   const u32 n = h.ntiles * (u32)(sizeof(TileDesc) / sizeof(uint2));
   for (u32 i = g; i < n; i += stride) tiles[i] = h_tiles[i];
 }
 
 // ------------------------------------------------------------------ Miller-Rabin
 
+// a * b * 2^-64 mod n (a, b < n, n odd), ninv = -n^{-1} mod 2^64: word-by-word
+// Montgomery with 32-bit multiplies (es_montgomery.h), 10 quarter-rate multiplies.
 __device__ __forceinline__ u64 mont_mul(u64 a, u64 b, u64 n, u64 ninv) {
-  const u64 lo = a * b, hi = __umul64hi(a, b);
-  const u64 m = lo * ninv;
-  const u64 mh = __umul64hi(m, n);
-  const u64 s = hi + mh;
-  const bool c1 = s < hi;
-  const u64 s2 = s + (lo != 0 ? 1ull : 0ull);
-  const bool c2 = s2 < s;
-  return (c1 || c2 || s2 >= n) ? s2 - n : s2;
+  return mont::mul(a, b, n, (u32)ninv);
 }
 
 __device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 n) {
@@ -612,24 +610,92 @@ struct RhoShare {
   u32 gen, busy, done, pad;
 };
 
-// How one rho call takes part: solo (rs null), owner of the published split, or helper
-// of split `gen`. Walk constants start above cbase, so no two waves repeat a walk.
+// Grid-wide posts of long cofactor splits. A wave whose walks have not split n after a
+// few Brent windows posts n in a free slot; workgroups that found the survivor queue empty
+// run walks of their own on posted values and post the first factor they find, which the
+// owner checks (it must divide n) and takes. Correctness never depends on the board: every
+// factor is verified, a slot may be reused at any time, and each side only reads single
+// 64-bit words with relaxed device-scope atomics.
+struct RhoBoard {
+  u64 n;        // the value being split; 0 = free
+  u64 factor;   // a factor a helper found (0 = none)
+  u64 window;   // the owner's current Brent window: helpers join the longest-running split
+  u64 pad;
+};
+constexpr u32 kBoardSlots = 64;           // one per lane of a wave
+constexpr u32 kBoardHelpWaves = 4;        // helper waves per workgroup: one per SIMD
+constexpr u32 kPostWindowMont = 64;       // Brent window at which a Montgomery split is posted
+constexpr u32 kPostWindowF64 = 256;       // ... and an FP64 one (its steps are cheaper)
+
+__device__ __forceinline__ u64 load_dev(const u64 *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u32 load_dev32(const u32 *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Lane 0's value for the whole wave. __builtin_amdgcn_readfirstlane returns an int: each
+// 32-bit half goes back through u32 before widening, or a half with its top bit set would
+// be sign-extended into the other.
+__device__ __forceinline__ u32 uniform32(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ u64 uniform64(u64 v) {
+  return (u64)uniform32((u32)v) | ((u64)uniform32((u32)(v >> 32)) << 32);
+}
+
+// How one rho call takes part (helper): 0 = solo or owner of the workgroup's shared split
+// (rs), 1 = helper of the workgroup's split `gen`, 2 = helper of board slot `slot`. Walk
+// constants start above cbase, so no two waves repeat a walk. An owner posts its split
+// on the board (slot >= 0 once posted) after a few windows.
 struct RhoRole {
   RhoShare *rs;
   u32 helper, gen, cbase;
+  RhoBoard *board;          // null: no grid-wide posts
+  const u32 *done;          // board helpers stop once *done >= ndone (every survivor finished)
+  u32 ndone;
+  int slot;
 };
 
-// The per-batch exchange. Owner: a posted factor that divides n, else 0 (a stale post
-// from an earlier split is cleared). Helper: kRhoStop once its split is over.
+// Claims a free board slot for n, or -1 if none is free: each lane reads one slot, and
+// lane 0 tries up to four of the free ones, starting from a slot that depends on n so that
+// owners posting at the same time try different slots.
+__device__ __forceinline__ int board_post(RhoBoard *board, u64 n, u32 lane) {
+  u64 free = __ballot(load_dev(&board[lane].n) == 0);
+  int slot = -1;
+  if (lane == 0) {
+    const u32 rot = (u32)(n % kBoardSlots);
+    free = (free >> rot) | (rot ? free << (64 - rot) : 0);    // slot rot + bit
+    for (int tries = 0; tries < 4 && free && slot < 0; ++tries) {
+      const u32 b = (u32)__builtin_ctzll(free);
+      free &= free - 1;
+      const u32 t = (rot + b) % kBoardSlots;
+      if (atomicCAS((unsigned long long *)&board[t].n, 0ull, (unsigned long long)n) == 0ull) slot = (int)t;
+    }
+  }
+  return __builtin_amdgcn_readfirstlane(slot);
+}
+
+// The owner's periodic exchange. Returns a factor of n (posted by a helper of the board
+// or of the workgroup), kRhoStop for a helper whose split is over, else 0. A post that
+// does not divide n (stale, from an earlier value of the slot) is cleared.
 __device__ __forceinline__ u64 rho_coop(const RhoRole &ro, u64 n) {
+  if (ro.helper == 2)
+    return (uniform64(load_dev(&ro.board[ro.slot].n)) != n ||
+            __builtin_amdgcn_readfirstlane(load_dev32(ro.done)) >= ro.ndone)
+               ? kRhoStop
+               : 0;
+  if (ro.board && ro.slot >= 0) {
+    const u64 g = uniform64(load_dev(&ro.board[ro.slot].factor));
+    if (g) {
+      if (g > 1 && g < n && n % g == 0) return g;
+      atomicCAS((unsigned long long *)&ro.board[ro.slot].factor, (unsigned long long)g, 0ull);
+    }
+  }
   if (!ro.rs) return 0;
   volatile RhoShare *v = ro.rs;
   if (ro.helper)
     return (__builtin_amdgcn_readfirstlane(v->busy) == 0 || __builtin_amdgcn_readfirstlane(v->gen) != ro.gen)
                ? kRhoStop
                : 0;
-  const u64 g = __builtin_amdgcn_readfirstlane((u32)v->factor) |
-                ((u64)__builtin_amdgcn_readfirstlane((u32)(v->factor >> 32)) << 32);
+  const u64 g = uniform64(v->factor);
   if (g == 0) return 0;
   if (g > 1 && g < n && n % g == 0) return g;
   atomicCAS((unsigned long long *)&ro.rs->factor, (unsigned long long)g, 0ull);
@@ -638,11 +704,25 @@ __device__ __forceinline__ u64 rho_coop(const RhoRole &ro, u64 n) {
 
 // What a walk that found factor g returns: the owner or a solo wave keeps it, a helper
 // posts it (first post wins) and stops.
-__device__ __forceinline__ u64 rho_found(const RhoRole &ro, u64 g, u32 lane) {
+__device__ __forceinline__ u64 rho_found(const RhoRole &ro, u64 n, u64 g, u32 lane) {
   if (!ro.helper) return g;
+  if (ro.helper == 2) {   // posted only while the slot still holds n
+    if (lane == 0 && load_dev(&ro.board[ro.slot].n) == n)
+      atomicCAS((unsigned long long *)&ro.board[ro.slot].factor, 0ull, (unsigned long long)g);
+    return kRhoStop;
+  }
   if (lane == 0 && ro.rs->gen == ro.gen)
     atomicCAS((unsigned long long *)&ro.rs->factor, 0ull, (unsigned long long)g);
   return kRhoStop;
+}
+
+// An owner posts its split once its walks reach Brent window `at` without a factor, and
+// then keeps the slot's window current, so helpers can pick the split that has run longest.
+__device__ __forceinline__ void maybe_post(RhoRole &ro, u64 n, u32 r, u32 at, u32 lane) {
+  if (ro.helper || !ro.board) return;
+  if (ro.slot < 0 && r == at) ro.slot = board_post(ro.board, n, lane);
+  if (ro.slot >= 0 && lane == 0)
+    __hip_atomic_store(&ro.board[ro.slot].window, (u64)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Brent's rho with an independent walk per lane (x -> x^2 + c, c = cbase + lane + 1,
@@ -650,13 +730,14 @@ __device__ __forceinline__ u64 rho_found(const RhoRole &ro, u64 g, u32 lane) {
 // kRhoFail when every walk fails inside the budget (the caller then hands the prime to
 // the host), or kRhoStop once *stop drops below k (a smaller rung is settled; the result
 // is moot) or, for a helper, once its split is over.
-__device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const RhoRole &ro) {
+__device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, RhoRole &ro) {
   const u64 ninv = neg_inv(n), one = (0 - n) % n;
   const u64 c = small_to_mont(ro.cbase + lane + 1, one, n);
   auto f = [&](u64 v) { return add_mod(mont_mul(v, v, n, ninv), c, n); };
   u64 y = small_to_mont(2, one, n), x = y, ys = y, acc = one;
   bool alive = true;
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
+    maybe_post(ro, n, r, kPostWindowMont, lane);
     x = y;
     // Advance r steps, looking for a stop or a posted factor every 64.
     for (u32 i = 0; i < r; ++i) {
@@ -686,7 +767,7 @@ __device__ u64 wave_rho(u64 n, u32 lane, const volatile u32 *stop, u32 k, const 
         if (g == 1) alive = false;
       }
       const u64 found = __ballot(g != 1);
-      if (found) return rho_found(ro, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
+      if (found) return rho_found(ro, n, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
       if (__ballot(alive) == 0) return kRhoFail;
       if (stop && *stop < k) return kRhoStop;
       const u64 h = rho_coop(ro, n);
@@ -702,7 +783,7 @@ constexpr int kRhoWalks = 4;   // independent walks per lane (4 measured best of
 // wave_rho for n < 2^50 (same contract): 64 x kRhoWalks walks x -> x^2 + c with
 // c = cbase + lane + 1 + 64 w, in signed FP64 residues (|x| < n); only the gcd converts
 // to integers, and gcd(a, n) does not depend on which representative of a mod n it sees.
-__device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, const RhoRole &ro) {
+__device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, RhoRole &ro) {
   const double nd = (double)n, ninv = 1.0 / nd;
   double c[kRhoWalks], y[kRhoWalks], x[kRhoWalks], ys[kRhoWalks], acc[kRhoWalks];
   bool alive[kRhoWalks];
@@ -715,6 +796,7 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
   }
   auto f = [&](double v, double cc) { return mulmod_sgn(v, v, nd, ninv) + cc; };
   for (u32 r = 1; r <= (1u << 16); r <<= 1) {
+    maybe_post(ro, n, r, kPostWindowF64, lane);
 #pragma unroll
     for (int w = 0; w < kRhoWalks; ++w) x[w] = y[w];
     // Advance r steps, looking for a stop or a posted factor every 16.
@@ -777,7 +859,7 @@ __device__ u64 wave_rho_f64(u64 n, u32 lane, const volatile u32 *stop, u32 k, co
 #pragma unroll
       for (int w = 0; w < kRhoWalks; ++w) any |= alive[w];
       const u64 found = __ballot(g != 1);
-      if (found) return rho_found(ro, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
+      if (found) return rho_found(ro, n, __shfl(g, __ffsll((long long)found) - 1, 64), lane);
       if (__ballot(any) == 0) return kRhoFail;
       if (stop && *stop < k) return kRhoStop;
       const u64 h = rho_coop(ro, n);
@@ -819,7 +901,7 @@ constexpr int kFacOk = 0, kFacHost = 1, kFacStop = 2;
 // primes below 2^16 that divide it. kFacHost if the device declines, kFacStop if a
 // smaller rung settled meanwhile.
 __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &T, WaveFac &F, u32 lane,
-                           const volatile u32 *stop, u32 k, RhoShare *rs) {
+                           const volatile u32 *stop, u32 k, RhoShare *rs, RhoBoard *board) {
   if (lane == 0) { F.k = 0; F.bad = 0; }
   wave_sync();
   u64 n = a;
@@ -865,9 +947,13 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
         atomicAdd(&rs->gen, 1u);
       }
       wave_sync();
-      const RhoRole ro{own ? rs : nullptr, 0u, 0u, 0u};
+      RhoRole ro{own ? rs : nullptr, 0u, 0u, 0u, board, nullptr, 0u, -1};
       const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k, ro) : wave_rho(m, lane, stop, k, ro);
       if (own && lane == 0) atomicExch(&rs->busy, 0u);
+      if (ro.slot >= 0 && lane == 0) {    // the split is over: free the board slot
+        atomicExch((unsigned long long *)&board[ro.slot].factor, 0ull);
+        atomicExch((unsigned long long *)&board[ro.slot].n, 0ull);
+      }
       if (d == kRhoStop) return kFacStop;
       if (d == kRhoFail || top + 2 > 4) return kFacHost;
       stack[top++] = d;
@@ -950,19 +1036,50 @@ __device__ bool help_split(RhoShare *rs, u32 &seen, u32 wv, u32 lane) {
   const u32 gen = __builtin_amdgcn_readfirstlane(v->gen);
   if (!__builtin_amdgcn_readfirstlane(v->busy) || gen == seen) return false;
   seen = gen;
-  const u64 n = __builtin_amdgcn_readfirstlane((u32)v->n) |
-                ((u64)__builtin_amdgcn_readfirstlane((u32)(v->n >> 32)) << 32);
+  const u64 n = uniform64(v->n);
   if (__builtin_amdgcn_readfirstlane(v->gen) == gen && n > 1) {
-    const RhoRole ro{rs, 1u, gen, 64u * kRhoWalks * (wv + 1)};   // disjoint from every other wave's walks
+    RhoRole ro{rs, 1u, gen, 64u * kRhoWalks * (wv + 1), nullptr, nullptr, 0u, -1};   // disjoint walks
     (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
   }
   return true;
 }
 
+// A wave of a workgroup that found the survivor queue empty: walks on values other
+// workgroups posted until every survivor of the batch is finished. `me` numbers the
+// helper waves of the grid, so helpers start their slot scan at different slots and no
+// two of them (or any owner) share walk constants; each visit takes fresh constants.
+// Helpers leave once *done reaches ndone, which every owner's bounded work guarantees.
+__device__ void board_help(RhoBoard *board, const u32 *done, u32 ndone, u32 lane, u32 me) {
+  for (u32 visit = 0;; ++visit) {
+    if (__builtin_amdgcn_readfirstlane(load_dev32(done)) >= ndone) return;
+    // Each lane reads one slot; the wave joins the open split with the largest Brent
+    // window (the one that has run longest), ties going to the first slot at or after `me`.
+    const u64 mine = load_dev(&board[lane].n);
+    const u64 win = mine ? load_dev(&board[lane].window) : 0;
+    const u32 lg = win ? 32u - (u32)__builtin_clz((u32)min(win, (u64)0x80000000u)) : 0u;
+    u32 key = mine ? (lg << 6) | (63u - ((lane - me) & 63u)) : 0u;
+    u32 best = key;
+    for (int o = 32; o > 0; o >>= 1) best = max(best, (u32)__shfl_xor((int)best, o, 64));
+    if (!best) {
+      __builtin_amdgcn_s_sleep(8);
+      continue;
+    }
+    const u32 slot = (u32)__builtin_ctzll(__ballot(key == best));
+    const u64 n = uniform64(__shfl(mine, (int)slot, 64));
+    // Owner walks use constants below 64 kRhoWalks (kRungWaves + 1) < 4096; a visit's
+    // constants stay below 2^27 < n / 8 (posted values exceed 2^32), which keeps the FP64
+    // walk inside the bounds its product is exact for.
+    RhoRole ro{nullptr, 2u, 0u, 4096u + ((visit & 255u) * 2048u + (me & 2047u)) * (64u * kRhoWalks), board, done,
+               ndone, (int)slot};
+    (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
+  }
+}
+
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
-                                                          TrialTab T, RungOut *__restrict__ out, u32 *queue) {
+                                                          TrialTab T, RungOut *__restrict__ out, u32 *queue,
+                                                          RhoBoard *board, u32 *done) {
   __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -982,9 +1099,16 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     if (tid == 0) cur_idx[it & 1] = atomicAdd(queue, 1u);
     __syncthreads();
     const u32 idx = cur_idx[it & 1];
-    if (idx >= n_) break;
+    if (idx >= n_) {
+      // The queue is empty: help the splits other workgroups posted, one wave per SIMD.
+      if (board && wv < kBoardHelpWaves) board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv);
+      break;
+    }
     const u8 f0 = flags[idx];
-    if (f0 == kComposite) continue;
+    if (f0 == kComposite) {
+      if (tid == 0 && done) atomicAdd(done, 1u);
+      continue;
+    }
     const u64 p = vals[idx];
     u32 r0 = (u32)((4 - (p & 3)) & 3);
     if (r0 == 0) r0 = 4;
@@ -1058,7 +1182,10 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     }
     __syncthreads();
     if (mr_composite) {                    // uniform: read after the barrier
-      if (tid == 0) flags[idx] = kComposite;
+      if (tid == 0) {
+        flags[idx] = kComposite;
+        if (done) atomicAdd(done, 1u);
+      }
       __syncthreads();
       continue;
     }
@@ -1079,7 +1206,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
       if (r > 1 && ga != 1) continue;
       const u32 ns = nfac[k];
-      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k, &rs);
+      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k, &rs, board);
       if (fr == kFacStop) break;
       if (fr == kFacHost) {
         if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
@@ -1110,6 +1237,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
           if (wres[w].status == kRungFound && wres[w].r == r0 + 4 * found_k) res = wres[w];
       }
       out[idx] = res;
+      if (done) atomicAdd(done, 1u);
     }
     __syncthreads();
   }
@@ -1223,7 +1351,9 @@ struct Slot {
   u64 *d_one = nullptr;             // 2^64 mod n per survivor, from the base-2 pass
   dev::RungOut *d_rung = nullptr;   // stage D result per survivor
   size_t surv_cap = 0;
-  u32 *d_scalars = nullptr;   // [0] survivor count, [1] tile queue head, [2] stage-D queue head
+  u32 *d_scalars = nullptr;   // [0] survivor count, [1] tile queue head, [2] stage-D queue head,
+                              // [3] survivors stage D has finished
+  dev::RhoBoard *d_board = nullptr;   // stage D's posted cofactor splits
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
   // writes the results into (device-side addresses in the dh_* twins).
   u32 *h_count = nullptr;
@@ -1290,7 +1420,7 @@ DeviceCtx::~DeviceCtx() {
     for (Slot &b : m.slot) {
       for (void *p : {(void *)b.d_tiles, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
                       (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung,
-                      (void *)b.d_scalars})
+                      (void *)b.d_scalars, (void *)b.d_board})
         if (p) (void)hipFree(p);
       for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_tiles, (void *)b.h_tile_counts,
                       (void *)b.h_surv_p, (void *)b.h_surv_w, (void *)b.h_flags, (void *)b.h_rung})
@@ -1351,6 +1481,7 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
     HIP_TRY(hipMalloc(&b.d_scalars, 4 * sizeof(u32)));
+    HIP_TRY(hipMalloc(&b.d_board, dev::kBoardSlots * sizeof(dev::RhoBoard)));
     HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
   }
@@ -1489,7 +1620,8 @@ int DeviceCtx::enqueue_batch(int si) {
   const LaunchCfg &cfg = b.cfg;
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
-                                                        (uint2 *)b.d_tiles, b.d_scalars);
+                                                        (uint2 *)b.d_tiles, b.d_scalars, (u64 *)b.d_board,
+                                                        dev::kBoardSlots * 4u);
     HIP_TRY(hipGetLastError());
     // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
     // not by separate event packets around it.
@@ -1512,7 +1644,8 @@ int DeviceCtx::enqueue_batch(int si) {
     if (cfg.device_tail) {
       // k_rung finishes the Miller-Rabin bases of the pending values itself.
       dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, st>>>(
-          b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags, b.d_one, m.trial, b.d_rung, b.d_scalars + 2);
+          b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags, b.d_one, m.trial, b.d_rung, b.d_scalars + 2,
+          b.d_board, b.d_scalars + 3);
     } else {
       dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
                                                              b.d_one);
@@ -1722,21 +1855,24 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   uint8_t *d_f = nullptr;
   u32 *d_n = nullptr;
   dev::RungOut *d_o = nullptr;
+  dev::RhoBoard *d_b = nullptr;
   HIP_TRY(hipMalloc(&d_v, n * 8));
   HIP_TRY(hipMalloc(&d_f, n));
-  HIP_TRY(hipMalloc(&d_n, 8));
+  HIP_TRY(hipMalloc(&d_n, 16));
   HIP_TRY(hipMalloc(&d_o, n * sizeof(dev::RungOut)));
-  const u32 n32[2] = {(u32)n, 0u};   // count, queue head
+  HIP_TRY(hipMalloc(&d_b, dev::kBoardSlots * sizeof(dev::RhoBoard)));
+  const u32 n32[4] = {(u32)n, 0u, 0u, 0u};   // count, queue head, unused, survivors finished
   std::vector<uint8_t> ones(n, 1);
   HIP_TRY(hipMemcpyAsync(d_v, vals, n * 8, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemcpyAsync(d_f, ones.data(), n, hipMemcpyHostToDevice, m.stream));
-  HIP_TRY(hipMemcpyAsync(d_n, n32, 8, hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemcpyAsync(d_n, n32, 16, hipMemcpyHostToDevice, m.stream));
+  HIP_TRY(hipMemsetAsync(d_b, 0, dev::kBoardSlots * sizeof(dev::RhoBoard), m.stream));
   dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32[0], d_f, nullptr, m.trial, d_o,
-                                                                         d_n + 1);
+                                                                         d_n + 1, d_b, d_n + 3);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));
-  for (void *p : {(void *)d_v, (void *)d_f, (void *)d_n, (void *)d_o}) HIP_TRY(hipFree(p));
+  for (void *p : {(void *)d_v, (void *)d_f, (void *)d_n, (void *)d_o, (void *)d_b}) HIP_TRY(hipFree(p));
   return 0;
 }
 

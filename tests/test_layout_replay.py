@@ -103,3 +103,14 @@ def test_window_lane_spans(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True)
     out = subprocess.run([exe], stdout=subprocess.PIPE, check=True).stdout.decode()
     assert out.strip() == "span ok", out
+
+
+def test_device_montgomery_product_is_exact(tmp_path):
+    """The kernels' 64-bit Montgomery product (es_montgomery.h, compiled here for the CPU
+    from the same source) times 2^64 equals a*b mod n under 128-bit integer arithmetic,
+    with the result reduced, for edge moduli up to 2^64 - 1 and 3 million random cases."""
+    src = os.path.join(HERE, "replay", "montgomery_replay.cpp")
+    exe = str(tmp_path / "montgomery_replay")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True)
+    out = subprocess.run([exe, "3000000", "11"], stdout=subprocess.PIPE, check=True).stdout.decode()
+    assert out.strip() == "ok 3000000", out
