@@ -44,6 +44,8 @@ struct BatchResult {
   std::vector<RungResult> rung;       // per survivor, when stage D ran on the device
   double device_ms = 0;               // sieve + MR kernels, HIP events
   double sieve_ms = 0;                // sieve kernel alone
+  double mr_ms = 0;                   // base-2 Miller-Rabin kernel alone
+  double tail_kernel_ms = 0;          // stage D kernel (or the other MR bases) alone
   int replays = 0;                    // survivor-buffer overflows that forced a rerun
 };
 

@@ -1089,6 +1089,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   __shared__ RhoShare rs;
   __shared__ u32 mr_composite;
   __shared__ u32 cur_idx[2];
+  __shared__ u32 help_batch;
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
@@ -1100,8 +1101,18 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     __syncthreads();
     const u32 idx = cur_idx[it & 1];
     if (idx >= n_) {
-      // The queue is empty: help the splits other workgroups posted, one wave per SIMD.
-      if (board && wv < kBoardHelpWaves) board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv);
+      // The queue is empty: help the splits other workgroups post, one wave per SIMD.
+      // Only a batch with a prime p >= 2^52 can need a Montgomery split (A = (p + r) / 4
+      // is then at least 2^50); below that every split runs in FP64 and is fast alone,
+      // and helpers that stay resident would only crowd the kernels of the next batches.
+      if (!board) break;
+      if (tid == 0) help_batch = 0;
+      __syncthreads();
+      bool big = false;
+      for (u32 i = tid; i < n_; i += blockDim.x) big |= vals[i] >= (1ull << 52) && flags[i] != kComposite;
+      if (big) help_batch = 1;             // any writer writes the same value
+      __syncthreads();
+      if (help_batch && wv < kBoardHelpWaves) board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv);
       break;
     }
     const u8 f0 = flags[idx];
@@ -1334,6 +1345,7 @@ struct Slot {
   // The batches share only read-only tables, and no kernel waits on another workgroup.
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t ev4 = nullptr, ev5 = nullptr, ev6 = nullptr, ev7 = nullptr;   // k_mr_base2, stage D
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_ok = false;
@@ -1427,7 +1439,7 @@ DeviceCtx::~DeviceCtx() {
         if (p) (void)hipHostFree(p);
       if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
       if (b.graph) (void)hipGraphDestroy(b.graph);
-      for (hipEvent_t e : {b.ev0, b.ev1, b.ev2, b.ev3}) if (e) (void)hipEventDestroy(e);
+      for (hipEvent_t e : {b.ev0, b.ev1, b.ev2, b.ev3, b.ev4, b.ev5, b.ev6, b.ev7}) if (e) (void)hipEventDestroy(e);
     }
     for (hipEvent_t e : {m.pev0, m.pev1}) if (e) (void)hipEventDestroy(e);
     if (m.capture) (void)hipStreamDestroy(m.capture);
@@ -1477,6 +1489,10 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipEventCreate(&b.ev1));
     HIP_TRY(hipEventCreate(&b.ev2));
     HIP_TRY(hipEventCreate(&b.ev3));
+    HIP_TRY(hipEventCreate(&b.ev4));
+    HIP_TRY(hipEventCreate(&b.ev5));
+    HIP_TRY(hipEventCreate(&b.ev6));
+    HIP_TRY(hipEventCreate(&b.ev7));
     HIP_TRY(hipMalloc(&b.d_hdr, sizeof(dev::BatchHdr)));
     HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
@@ -1638,17 +1654,20 @@ int DeviceCtx::enqueue_batch(int si) {
                             (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
     }
     HIP_TRY(hipGetLastError());
-    dev::k_mr_base2<<<std::max(1, cus_ * 2), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
-                                                           b.d_one);
+    hipExtLaunchKernelGGL(dev::k_mr_base2, dim3(std::max(1, cus_ * 2)), dim3(256), 0, st, timed ? b.ev4 : nullptr,
+                          timed ? b.ev5 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
+                          (u32)b.surv_cap, b.d_flags, b.d_one);
     HIP_TRY(hipGetLastError());
     if (cfg.device_tail) {
       // k_rung finishes the Miller-Rabin bases of the pending values itself.
-      dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, st>>>(
-          b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags, b.d_one, m.trial, b.d_rung, b.d_scalars + 2,
-          b.d_board, b.d_scalars + 3);
+      hipExtLaunchKernelGGL(dev::k_rung, dim3(std::max(1, cus_)), dim3(64 * dev::kRungWaves), 0, st,
+                            timed ? b.ev6 : nullptr, timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p,
+                            (const u32 *)b.d_scalars, (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, m.trial,
+                            b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3);
     } else {
-      dev::k_mr_bases<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.d_surv_p, b.d_scalars, (u32)b.surv_cap, b.d_flags,
-                                                             b.d_one);
+      hipExtLaunchKernelGGL(dev::k_mr_bases, dim3(std::max(1, cus_ * 4)), dim3(256), 0, st, timed ? b.ev6 : nullptr,
+                            timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
+                            (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one);
     }
     HIP_TRY(hipGetLastError());
     dev::k_pack<<<std::max(1, cus_), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
@@ -1798,16 +1817,22 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
   out.surv_prime.clear();
   out.rung.clear();
   out.replays = 0;
-  out.device_ms = out.sieve_ms = 0;
+  out.device_ms = out.sieve_ms = out.mr_ms = out.tail_kernel_ms = 0;
   if (b.nt == 0) return 0;
   const bool use_graph = b.cfg.graph && b.cfg.persistent;
   for (;;) {
     HIP_TRY(hipEventSynchronize(b.ev3));
-    float all = 0, sv = 0;
+    float all = 0, sv = 0, mr = 0, rg = 0;
     HIP_TRY(hipEventElapsedTime(&all, b.ev0, b.ev3));
-    if (!use_graph) HIP_TRY(hipEventElapsedTime(&sv, b.ev1, b.ev2));
+    if (!use_graph) {
+      HIP_TRY(hipEventElapsedTime(&sv, b.ev1, b.ev2));
+      HIP_TRY(hipEventElapsedTime(&mr, b.ev4, b.ev5));
+      HIP_TRY(hipEventElapsedTime(&rg, b.ev6, b.ev7));
+    }
     out.device_ms += all;
     out.sieve_ms += use_graph ? -1.0 : sv;
+    out.mr_ms += use_graph ? -1.0 : mr;
+    out.tail_kernel_ms += use_graph ? -1.0 : rg;
     const u32 cnt = *b.h_count;
     if (cnt <= b.surv_cap) break;
     // Overflow: the counts are complete, the list is not. Grow and replay the batch

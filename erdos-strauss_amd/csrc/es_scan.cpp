@@ -442,6 +442,8 @@ int Scanner::device_collect(std::vector<Interval> &wins, std::vector<WindowRecor
   tm.batch_ms += t2 - t1;
   tm.device_ms = br.device_ms;
   tm.sieve_ms = br.sieve_ms;
+  tm.mr_ms = br.mr_ms;
+  tm.rung_ms = br.tail_kernel_ms;
   tm.replays = br.replays;
   tm.survivors = br.surv_p.size();
 

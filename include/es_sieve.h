@@ -47,11 +47,14 @@ typedef struct es_counts {
   uint64_t positions, covered, sieved, mr, composite, rung, direct, survivors;
 } es_counts;
 
-/* Timing of the last scan call, milliseconds (scan.cu:150-162's --profile split). */
+/* Timing of the last scan call, milliseconds (scan.cu:150-162's --profile split).
+ * sieve_ms, mr_ms and rung_ms time the marking, base-2 Miller-Rabin and stage-D kernels
+ * alone (HIP events on their dispatches); a batch replayed from a graph reports -1. */
 typedef struct es_timing {
   double plan_ms, batch_ms, device_ms, sieve_ms, tail_ms, total_ms;
   uint64_t tiles, survivors, primes, host_primes;
   int replays;
+  double mr_ms, rung_ms;
 } es_timing;
 
 /* Fills *cfg with rung_scan3's defaults. */
