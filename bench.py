@@ -29,7 +29,8 @@ Also reported, in the same JSON line:
                at a time and four in flight; and the MI355X-tuned filter table
   c5           the serving trace (Poisson 200 req/s, seed 1) replayed in real time through
                the continuous-batching server, every request's record checked against its
-               committed sha256
+               committed sha256; then the whole trace at 10x, 40x and 80x its arrival rate, the
+               server near saturation
   cpu_baseline the reference scanner built from its own sources (oracle/_ref/rung_scan3)
                on a bounded sample of the batch's windows, on this host's cores, startup
                and work separated; cpu_baseline_port: the oracle over all 512 windows
@@ -280,6 +281,26 @@ def tuned_table_bench(es, steps):
     return out
 
 
+def c5_compressed(es, factor):
+    """The whole trace with its arrival times divided by `factor` (factor x the offered
+    load): the server near or past saturation. Every record is checked against its digest."""
+    rows = [l.split() for l in open(os.path.join(GOLDEN, "c5_trace.txt")) if not l.startswith("#")]
+    digests = [l.split()[0] for l in open(os.path.join(GOLDEN, "c5", "records_sha256.txt")) if not l.startswith("#")]
+    reqs = [(float(a) / factor, int(lo), int(hi)) for a, lo, hi in rows]
+    sc = es.Scanner(wheel=2042040, filters=es.F204, device=0)
+    sc.autotune([(lo, min(hi, lo + (1 << 32))) for _, lo, hi in reqs[:64]])
+    recs, infos, st = es.serve_trace(sc, reqs, realtime=True, hbm_fraction=0.8)
+    sc.close()
+    bad = sum(1 for i, r in enumerate(recs) if hashlib.sha256(r.encode()).hexdigest() != digests[i])
+    lat = [i["done_ms"] - i["arrival_ms"] for i in infos]
+    total = sum(hi - lo for _, lo, hi in reqs)
+    return {"arrival_rate_req_s": 200.0 * factor, "offered": total / (reqs[-1][0] or 1e-9),
+            "sustained": total / (st["wall_ms"] * 1e-3), "requests": len(recs), "records_identical": bad == 0,
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+            "device_busy_frac": st["busy_ms"] / st["wall_ms"], "max_active": st["max_active"],
+            "max_batch_requests": st["max_batch_requests"], "batches": st["batches"], "wall_ms": st["wall_ms"]}
+
+
 def c5_bench(es, seconds):
     """The serving trace replayed in real time for its first `seconds` through the
     continuous-batching server (wheel 2042040, the GPU filter table, admission budget 80%
@@ -472,6 +493,7 @@ def main():
             out["legs"]["tuned_table"] = tuned_table_bench(es, args.steps)
     if args.c5_seconds > 0:
         out["c5"] = c5_bench(es, args.c5_seconds)
+        out["c5"]["compressed"] = [c5_compressed(es, f) for f in (10.0, 40.0, 80.0)]
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(cores)

@@ -115,3 +115,25 @@ def test_streaming_api_refusal_and_empty(sc, golden):
             srv.wait(10 ** 9, timeout_ms=1)             # unknown id
         st = srv.stats()
         assert st["completed"] == len(ids) + 2 and st["refused"] == 1
+
+
+def test_realtime_first_20s_every_record_matches_digest(sc):
+    """The first 20 s of the trace (about 3,900 requests, 2.6e12 integers) replayed in real
+    time through the continuous-batching server: every request's record hashes to the
+    committed sha256 of the oracle's record for that request (tests/golden/c5/
+    records_sha256.txt, 1,018 of whose requests were also run through the reference
+    scanner), with requests overlapping on the device."""
+    import hashlib
+    rows = [l.split() for l in open(os.path.join(paths.GOLDEN, "c5_trace.txt")) if not l.startswith("#")]
+    digests = [l.split() for l in open(os.path.join(paths.GOLDEN, "c5", "records_sha256.txt"))
+               if not l.startswith("#")]
+    reqs = [(float(a), int(lo), int(hi)) for a, lo, hi in rows if float(a) < 20.0]
+    assert len(reqs) > 3000
+    print("replaying", len(reqs), "requests in real time", flush=True)
+    recs, infos, st = es.serve_trace(sc, reqs, realtime=True, hbm_fraction=0.8)
+    bad = [i for i, r in enumerate(recs)
+           if hashlib.sha256(r.encode()).hexdigest() != digests[i][0] or int(digests[i][1]) != reqs[i][1]]
+    print("requests", len(recs), "differing", len(bad), "batches", st["batches"], "max active", st["max_active"],
+          flush=True)
+    assert not bad, bad[:5]
+    assert st["completed"] == len(reqs) and st["max_batch_requests"] > 1
