@@ -624,8 +624,8 @@ struct RhoBoard {
 };
 constexpr u32 kBoardSlots = 64;           // one per lane of a wave
 constexpr u32 kBoardHelpWaves = 4;        // helper waves per workgroup: one per SIMD
-constexpr u32 kPostWindowMont = 64;       // Brent window at which a Montgomery split is posted
-constexpr u32 kPostWindowF64 = 256;       // ... and an FP64 one (its steps are cheaper)
+constexpr u32 kPostWindowMont = 32;       // Brent window at which a Montgomery split is posted
+constexpr u32 kPostWindowF64 = 128;       // ... and an FP64 one (its steps are cheaper)
 
 __device__ __forceinline__ u64 load_dev(const u64 *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -716,11 +716,12 @@ __device__ __forceinline__ u64 rho_found(const RhoRole &ro, u64 n, u64 g, u32 la
   return kRhoStop;
 }
 
-// An owner posts its split once its walks reach Brent window `at` without a factor, and
-// then keeps the slot's window current, so helpers can pick the split that has run longest.
+// An owner posts its split once its walks reach Brent window `at` without a factor (and
+// tries again at each later window while the board is full), then keeps the slot's window
+// current, so helpers can pick the split that has run longest.
 __device__ __forceinline__ void maybe_post(RhoRole &ro, u64 n, u32 r, u32 at, u32 lane) {
   if (ro.helper || !ro.board) return;
-  if (ro.slot < 0 && r == at) ro.slot = board_post(ro.board, n, lane);
+  if (ro.slot < 0 && r >= at) ro.slot = board_post(ro.board, n, lane);   // again each window if full
   if (ro.slot >= 0 && lane == 0)
     __hip_atomic_store(&ro.board[ro.slot].window, (u64)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
