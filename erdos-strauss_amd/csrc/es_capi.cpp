@@ -87,18 +87,22 @@ static int records_text(es_scanner *s, const std::vector<es::WindowRecord> &recs
                         size_t *text_len, size_t *rec_len, int *exit_code) {
   const size_t nwin = recs.size();
   std::string all;
+  size_t guess = 0;
+  for (const es::WindowRecord &r : recs) guess += 480 + r.lines.size() * 64;
+  all.reserve(guess);
   for (size_t i = 0; i < nwin; ++i) {
-    const std::string r = s->sc.record_text(recs[i], emit_survivors != 0);
-    if (rec_len) rec_len[i] = r.size();
+    const size_t before = all.size();
+    s->sc.append_record(all, recs[i], emit_survivors != 0);
+    if (rec_len) rec_len[i] = all.size() - before;
     if (exit_code) exit_code[i] = recs[i].survivors ? 2 : 0;
-    all += r;
   }
-  char *buf = (char *)malloc(all.size() + 1);
+  const size_t total = all.size();
+  char *buf = (char *)malloc(total + 1);
   if (!buf) return fail(ES_ERR_ARG, "out of host memory");
-  memcpy(buf, all.data(), all.size());
-  buf[all.size()] = 0;
+  memcpy(buf, all.data(), total);
+  buf[total] = 0;
   *text = buf;
-  if (text_len) *text_len = all.size();
+  if (text_len) *text_len = total;
   return ES_OK;
 }
 

@@ -113,12 +113,14 @@ class DeviceCtx {
   int grow_surv(int slot, size_t need);
   int enqueue_batch(int slot);
   int read_batch(int slot, BatchResult &out);
+  int wait_slot(int slot);
   std::string err_;
   int device_ = -1;
   int cus_ = 0;
   size_t lds_limit_ = 0;   // dynamic LDS one workgroup may allocate (bytes)
   size_t layout_bytes_ = 0;
   void *scratch_ = nullptr;   // this device's copy of the scratch array (device address)
+  double clock_ms_per_tick_ = 1e-5;   // the device's constant clock (s_memrealtime)
 };
 
 }  // namespace es

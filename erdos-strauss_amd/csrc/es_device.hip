@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -58,8 +59,12 @@ struct LayoutDev {
   u32 P, Pmagic, Pc32, nl, nf, ncov, mask_words, ostride;
 };
 
+// Per-batch scalars: survivor count, tile queue, stage-D queue, survivors stage D has
+// finished, workgroups of k_pack done.
+constexpr u32 kScalars = 5;
+
 struct BatchHdr {
-  u32 ntiles, nwin, has_sieve, pad;
+  u32 ntiles, nwin, has_sieve, seq;   // seq: the batch's number, posted back when it completes
 };
 
 // x mod s for s < 2^16 with mg = floor(2^32 / s) (2^32 - 1 for s = 1): the quotient
@@ -313,11 +318,15 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
 // capacity. Every other per-batch output is written in full by the kernels.
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
                         BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars,
-                        u64 *__restrict__ board_words, u32 nboard_words) {
+                        u64 *__restrict__ board_words, u32 nboard_words, u64 *__restrict__ h_sync) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const BatchHdr h = *h_hdr;
-  if (g == 0) *hdr = h;
-  if (g < 4) scalars[g] = 0;   // survivor count, tile queue, stage-D queue, survivors finished
+  if (g == 0) {
+    *hdr = h;
+    h_sync[1] = __builtin_amdgcn_s_memrealtime();   // the batch's start on the device clock
+  }
+  if (g < kScalars) scalars[g] = 0;   // survivor count, tile queue, stage-D queue, survivors finished,
+                                      // workgroups of k_pack done
   for (u32 i = g; i < nboard_words; i += stride) board_words[i] = 0;   // stage D's board of posted splits
   const u32 n = h.ntiles * (u32)(sizeof(TileDesc) / sizeof(uint2));
   for (u32 i = g; i < n; i += stride) tiles[i] = h_tiles[i];
@@ -1259,11 +1268,15 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
 // (zero-copy), so the host needs one stream synchronisation and no staging copies:
 // the survivor count, the per-tile counters, and per survivor p, window, primality flag
 // and (primes only) the stage-D result.
+// The last of its workgroups to finish posts the batch's end time and number into pinned
+// host memory (h_sync[2], h_sync[0]) after system-scope fences, so the host sees the
+// batch complete, with every mirror written, by reading host memory alone.
 __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict__ tile_counts,
                        const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
                        const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
                        const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, uint4 *h_tc,
-                       u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r) {
+                       u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r, u32 *__restrict__ done_blocks,
+                       u64 *h_sync) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const u32 nt = hdr->ntiles, n = min(*surv_count, cap);
   if (g == 0) h_count[0] = *surv_count;
@@ -1274,6 +1287,14 @@ __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict
     h_w[i] = surv_w[i];
     h_f[i] = f;
     if (with_rung && f) h_r[i] = rung[i];
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(done_blocks, 1u) == gridDim.x - 1) {
+    __threadfence_system();
+    h_sync[2] = __builtin_amdgcn_s_memrealtime();
+    __threadfence_system();
+    __hip_atomic_store(&h_sync[0], (u64)hdr->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1370,6 +1391,9 @@ struct Slot {
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
   // writes the results into (device-side addresses in the dh_* twins).
   u32 *h_count = nullptr;
+  u64 *h_sync = nullptr;            // [0] number of the last batch completed, [1] its start, [2] its end
+  u64 *dh_sync = nullptr;
+  u32 seq = 0;                      // number of the batch in flight in this slot
   TileDesc *h_tiles = nullptr;
   TileDesc *dh_tiles = nullptr;
   dev::BatchHdr *dh_hdr = nullptr;
@@ -1435,7 +1459,7 @@ DeviceCtx::~DeviceCtx() {
                       (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung,
                       (void *)b.d_scalars, (void *)b.d_board})
         if (p) (void)hipFree(p);
-      for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_tiles, (void *)b.h_tile_counts,
+      for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_sync, (void *)b.h_tiles, (void *)b.h_tile_counts,
                       (void *)b.h_surv_p, (void *)b.h_surv_w, (void *)b.h_flags, (void *)b.h_rung})
         if (p) (void)hipHostFree(p);
       if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
@@ -1478,6 +1502,9 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipGetDeviceProperties(&p, device));
   cus_ = p.multiProcessorCount;
   lds_limit_ = p.sharedMemPerBlock;
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+  clock_ms_per_tick_ = khz > 0 ? 1.0 / khz : 1e-5;
   Impl &m = *impl_;
   HIP_TRY(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
   m.slot[0].stream = m.stream;
@@ -1497,7 +1524,10 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipMalloc(&b.d_hdr, sizeof(dev::BatchHdr)));
     HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
-    HIP_TRY(hipMalloc(&b.d_scalars, 4 * sizeof(u32)));
+    HIP_TRY(hipMalloc(&b.d_scalars, dev::kScalars * sizeof(u32)));
+    HIP_TRY(hipHostMalloc(&b.h_sync, 4 * sizeof(u64), hipHostMallocMapped));
+    HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_sync, b.h_sync, 0));
+    std::memset(b.h_sync, 0, 4 * sizeof(u64));
     HIP_TRY(hipMalloc(&b.d_board, dev::kBoardSlots * sizeof(dev::RhoBoard)));
     HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
@@ -1638,7 +1668,7 @@ int DeviceCtx::enqueue_batch(int si) {
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
                                                         (uint2 *)b.d_tiles, b.d_scalars, (u64 *)b.d_board,
-                                                        dev::kBoardSlots * 4u);
+                                                        dev::kBoardSlots * 4u, b.dh_sync);
     HIP_TRY(hipGetLastError());
     // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
     // not by separate event packets around it.
@@ -1674,12 +1704,14 @@ int DeviceCtx::enqueue_batch(int si) {
     dev::k_pack<<<std::max(1, cus_), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
                                                    b.d_surv_p, b.d_surv_w, b.d_flags, b.d_rung,
                                                    cfg.device_tail ? 1u : 0u, b.dh_count, b.dh_tile_counts,
-                                                   b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung);
+                                                   b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung, b.d_scalars + 4,
+                                                   b.dh_sync);
     HIP_TRY(hipGetLastError());
     return 0;
   };
-  HIP_TRY(hipEventRecord(b.ev0, b.stream));
-  if (cfg.graph && cfg.persistent) {
+  const bool graphed = cfg.graph && cfg.persistent;
+  if (!graphed) HIP_TRY(hipEventRecord(b.ev0, b.stream));
+  if (graphed) {
     if (!b.graph_ok || b.graph_block != cfg.block || b.graph_lds != b.lds || b.graph_tail != cfg.device_tail ||
         b.graph_grid != b.grid || b.graph_off_words != b.off_words) {
       if (b.graph_exec) { HIP_TRY(hipGraphExecDestroy(b.graph_exec)); b.graph_exec = nullptr; }
@@ -1700,10 +1732,10 @@ int DeviceCtx::enqueue_batch(int si) {
       b.graph_off_words = b.off_words;
     }
     HIP_TRY(hipGraphLaunch(b.graph_exec, b.stream));
-  } else if (enqueue(b.stream, true)) {
-    return -1;
+  } else {
+    if (enqueue(b.stream, true)) return -1;
+    HIP_TRY(hipEventRecord(b.ev3, b.stream));
   }
-  HIP_TRY(hipEventRecord(b.ev3, b.stream));
   return 0;
 }
 
@@ -1749,7 +1781,7 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     b.h_hdr->ntiles = (u32)nt;
     b.h_hdr->nwin = nwin;
     b.h_hdr->has_sieve = has_sieve ? 1 : 0;
-    b.h_hdr->pad = 0;
+    b.h_hdr->seq = ++b.seq;
     std::memcpy(b.h_tiles, tiles.data(), nt * sizeof(TileDesc));   // k_stage copies both in
 
     // One LDS byte per position, sized to the batch's largest tile (rounded to whole
@@ -1806,6 +1838,28 @@ int DeviceCtx::finish(BatchResult &out) {
   return rc;
 }
 
+// Waits until k_pack has posted slot si's batch number into pinned host memory: no HIP
+// call while the batch runs, so another thread may launch batches meanwhile. Every
+// millisecond the stream is asked for an error, so a failed batch cannot hang the wait.
+int DeviceCtx::wait_slot(int si) {
+  Slot &b = impl_->slot[si];
+  const u64 want = b.seq;
+  auto t_next = std::chrono::steady_clock::now() + std::chrono::milliseconds(1);
+  for (;;) {
+    if (__atomic_load_n(&b.h_sync[0], __ATOMIC_ACQUIRE) == want) return 0;
+    __builtin_ia32_pause();
+    if (std::chrono::steady_clock::now() < t_next) continue;
+    t_next += std::chrono::milliseconds(1);
+    const hipError_t e = hipStreamQuery(b.stream);
+    if (e == hipSuccess) {
+      if (__atomic_load_n(&b.h_sync[0], __ATOMIC_ACQUIRE) == want) return 0;
+      err_ = "batch finished without posting its completion";
+      return -1;
+    }
+    if (e != hipErrorNotReady) { err_ = std::string("batch failed: ") + hipGetErrorString(e); return -1; }
+  }
+}
+
 // Waits for slot si's batch (only that batch: later ones keep running), replays it on a
 // survivor-list overflow, and copies its results out of the pinned mirrors.
 int DeviceCtx::read_batch(int si, BatchResult &out) {
@@ -1822,15 +1876,14 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
   if (b.nt == 0) return 0;
   const bool use_graph = b.cfg.graph && b.cfg.persistent;
   for (;;) {
-    HIP_TRY(hipEventSynchronize(b.ev3));
-    float all = 0, sv = 0, mr = 0, rg = 0;
-    HIP_TRY(hipEventElapsedTime(&all, b.ev0, b.ev3));
+    if (wait_slot(si)) return -1;
+    float sv = 0, mr = 0, rg = 0;
     if (!use_graph) {
       HIP_TRY(hipEventElapsedTime(&sv, b.ev1, b.ev2));
       HIP_TRY(hipEventElapsedTime(&mr, b.ev4, b.ev5));
       HIP_TRY(hipEventElapsedTime(&rg, b.ev6, b.ev7));
     }
-    out.device_ms += all;
+    out.device_ms += (double)(b.h_sync[2] - b.h_sync[1]) * clock_ms_per_tick_;
     out.sieve_ms += use_graph ? -1.0 : sv;
     out.mr_ms += use_graph ? -1.0 : mr;
     out.tail_kernel_ms += use_graph ? -1.0 : rg;
@@ -1839,9 +1892,11 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
     // Overflow: the counts are complete, the list is not. Grow and replay the batch
     // (sieve/cuda/scan.cu:223-242 does the same).
     ++out.replays;
-    if (grow_surv(si, cnt) || enqueue_batch(si)) return -1;
+    if (grow_surv(si, cnt)) return -1;
+    b.h_hdr->seq = ++b.seq;
+    if (enqueue_batch(si)) return -1;
   }
-  // Everything is in the pinned mirrors now (k_pack ran before ev3).
+  // Everything is in the pinned mirrors now (k_pack posted the batch's number last).
   const u32 cnt = *b.h_count;
   out.surv_p.assign(b.h_surv_p, b.h_surv_p + cnt);
   out.surv_w.assign(b.h_surv_w, b.h_surv_w + cnt);

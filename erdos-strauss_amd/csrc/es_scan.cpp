@@ -7,6 +7,7 @@
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <numeric>
@@ -130,7 +131,15 @@ void shard_interval(u64 &lo, u64 &hi, u64 shard, u64 nshard) {
 }
 
 int Scanner::init_wheel(u64 M, u64 mmax, std::string &msg) {
-  return build_plan(M, mmax, plan_, msg);
+  if (build_plan(M, mmax, plan_, msg)) return 1;
+  char b[320];
+  snprintf(b, sizeof b,
+           " wheelM=%" PRIu64 " lanes=%zu mmax=%" PRIu64 " ucert_rmax=%u ucert_umax=%u wheel_pmin=%" PRIu64
+           " classes_sha256=%s",
+           plan_.M, plan_.wheel.classes.size(), plan_.mmax, kUcertRmax, kUcertUmax, plan_.wheel.max_pmin,
+           plan_.pin ? plan_.pin->sha256 : "unpinned");
+  summary_mid_ = b;
+  return 0;
 }
 
 int Scanner::init_filters(const char *filter_file, u64 sieve, std::string &msg) {
@@ -601,20 +610,53 @@ int Scanner::warm_layouts(std::string &msg) {
   return 0;
 }
 
-std::string Scanner::record_text(const WindowRecord &r, bool emit_survivors) const {
-  std::string s;
+void Scanner::append_record(std::string &s, const WindowRecord &r, bool emit_survivors) const {
   for (const std::string &l : r.lines)
     if (emit_survivors || l.compare(0, 9, "SURVIVOR ") != 0) s += l;
+  // SUMMARY lo=.. hi=.. <the scanner's constant fields> positions=.. ... survivors=..
+  // (rung_scan3.cpp:1153-1161): the same text as the printf format, written into a stack
+  // buffer with put_u64 instead of parsing a format string per record.
   char b[640];
-  snprintf(b, sizeof b,
-           "SUMMARY lo=%" PRIu64 " hi=%" PRIu64 " wheelM=%" PRIu64 " lanes=%zu mmax=%" PRIu64
-           " ucert_rmax=%u ucert_umax=%u wheel_pmin=%" PRIu64 " classes_sha256=%s positions=%" PRIu64
-           " covered=%" PRIu64 " sieved=%" PRIu64 " mr=%" PRIu64 " composite=%" PRIu64 " rung=%" PRIu64
-           " direct=%" PRIu64 " survivors=%" PRIu64 "\n",
-           r.lo, r.hi, plan_.M, plan_.wheel.classes.size(), plan_.mmax, kUcertRmax, kUcertUmax,
-           plan_.wheel.max_pmin, plan_.pin ? plan_.pin->sha256 : "unpinned", r.positions, r.covered, r.sieved,
-           r.mr, r.composite, r.rung, r.direct, r.survivors);
-  s += b;
+  char *w = b;
+  auto lit = [&](const char *k, size_t n) {
+    memcpy(w, k, n);
+    w += n;
+  };
+  auto num = [&](u64 v) {
+    char t[24];
+    char *e = t + sizeof t;
+    const char *d = put_u64(e, v);
+    memcpy(w, d, e - d);
+    w += e - d;
+  };
+  lit("SUMMARY lo=", 11);
+  num(r.lo);
+  lit(" hi=", 4);
+  num(r.hi);
+  lit(summary_mid_.data(), summary_mid_.size());
+  lit(" positions=", 11);
+  num(r.positions);
+  lit(" covered=", 9);
+  num(r.covered);
+  lit(" sieved=", 8);
+  num(r.sieved);
+  lit(" mr=", 4);
+  num(r.mr);
+  lit(" composite=", 11);
+  num(r.composite);
+  lit(" rung=", 6);
+  num(r.rung);
+  lit(" direct=", 8);
+  num(r.direct);
+  lit(" survivors=", 11);
+  num(r.survivors);
+  *w++ = '\n';
+  s.append(b, w - b);
+}
+
+std::string Scanner::record_text(const WindowRecord &r, bool emit_survivors) const {
+  std::string s;
+  append_record(s, r, emit_survivors);
   return s;
 }
 

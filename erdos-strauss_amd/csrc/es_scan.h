@@ -101,6 +101,8 @@ class Scanner {
 
   // The full record text: lines (SURVIVOR lines dropped if !emit_survivors) + SUMMARY.
   std::string record_text(const WindowRecord &r, bool emit_survivors) const;
+  // The same text appended to s.
+  void append_record(std::string &s, const WindowRecord &r, bool emit_survivors) const;
 
   const Plan &plan() const { return plan_; }
   Tuning &tuning() { return tune_; }
@@ -136,6 +138,7 @@ class Scanner {
   std::vector<Estimate> est_;
   std::map<u64, std::unique_ptr<Layout>> layouts_;
   std::map<u64, int> layout_id_;
+  std::string summary_mid_;   // SUMMARY's fields that do not depend on the window
 };
 
 // Splits [lo, hi) into shard i of n the way rung_scan3.cpp:1022-1027 does.

@@ -12,6 +12,29 @@
 
 namespace es {
 
+// Decimal digits of v written backwards ending at `end`, two at a time from a table
+// (64-bit division by a constant compiles to a multiply); returns the first digit.
+char *put_u64(char *end, u64 v) {
+  static const char kPairs[] =
+      "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
+      "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
+      "8081828384858687888990919293949596979899";
+  while (v >= 100) {
+    const u64 q = v / 100;
+    const unsigned r = (unsigned)(v - q * 100);
+    end -= 2;
+    memcpy(end, kPairs + 2 * r, 2);
+    v = q;
+  }
+  if (v >= 10) {
+    end -= 2;
+    memcpy(end, kPairs + 2 * v, 2);
+  } else {
+    *--end = (char)('0' + v);
+  }
+  return end;
+}
+
 namespace {
 
 constexpr u64 kTrialBound = 65536;   // census/rung_scan.cpp:175
@@ -213,29 +236,6 @@ u128 least_witness(u64 p, const Fac &fa, u128 q, unsigned r, unsigned target) {
   absorb(p, 2);
   for (int i = 0; i < fa.k; ++i) absorb(fa.p[i], 2 * fa.e[i]);
   return best[target] == NONE ? 0 : best[target];
-}
-
-// Decimal digits of v written backwards ending at `end`, two at a time from a table
-// (64-bit division by a constant compiles to a multiply); returns the first digit.
-char *put_u64(char *end, u64 v) {
-  static const char kPairs[] =
-      "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
-      "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
-      "8081828384858687888990919293949596979899";
-  while (v >= 100) {
-    const u64 q = v / 100;
-    const unsigned r = (unsigned)(v - q * 100);
-    end -= 2;
-    memcpy(end, kPairs + 2 * r, 2);
-    v = q;
-  }
-  if (v >= 10) {
-    end -= 2;
-    memcpy(end, kPairs + 2 * v, 2);
-  } else {
-    *--end = (char)('0' + v);
-  }
-  return end;
 }
 
 // As put_u64 for 128-bit v: one 128-bit division splits off the low 19 digits.

@@ -30,6 +30,9 @@ bool solve_direct(u64 p, std::string &line);
 // 0 = RUNG, 1 = SOLVED, 2 = SURVIVOR; line gets the certificate text.
 int certify_prime(u64 p, std::string &line);
 
+// Decimal digits of v written backwards so that they end at `end`; returns the first.
+char *put_u64(char *end, u64 v);
+
 // "RUNG p A r u\n" (rung_scan3.cpp:594-600).
 void format_rung(u64 p, u64 A, unsigned r, u128 u, std::string &line);
 

@@ -4,6 +4,7 @@ Mirrors the reference's scanner interface at the process level (rung_scan3's fla
 records and exit codes) as Python calls. There is no CPU fallback: opening a scanner
 without a HIP device raises EsError.
 """
+import array
 import ctypes
 import os
 
@@ -150,8 +151,22 @@ def certify(p):
     return kind, buf.value.decode()
 
 
+def _split_text(raw, rl, n):
+    """Records as str, cut by their byte lengths (records are ASCII: one decode, then
+    slices by the same offsets)."""
+    txt = raw.decode("ascii")
+    out, pos = [], 0
+    for i in range(n):
+        k = rl[i]
+        out.append(txt[pos:pos + k])
+        pos += k
+    return out
+
+
 def _u64s(vals):
-    return (U64 * len(vals))(*vals)
+    # array('Q') packs a list of Python ints far faster than the ctypes array constructor.
+    buf = array.array("Q", vals)
+    return (U64 * len(vals)).from_buffer(buf) if vals else (U64 * 0)()
 
 
 MAX_IN_FLIGHT = 4   # ES_MAX_IN_FLIGHT of include/es_sieve.h
@@ -209,11 +224,7 @@ class Scanner:
             raw = ctypes.string_at(text, tl.value)
         finally:
             lib().es_free(text)
-        out, pos = [], 0
-        for i in range(n):
-            out.append(raw[pos:pos + rl[i]].decode())
-            pos += rl[i]
-        return out, [ec[i] for i in range(n)]
+        return _split_text(raw, rl, n), [ec[i] for i in range(n)]
 
     def submit(self, windows):
         """Pipelined records(): plans [(lo, hi)] and launches the batch without waiting.
@@ -242,11 +253,7 @@ class Scanner:
             raw = ctypes.string_at(text, tl.value)
         finally:
             lib().es_free(text)
-        out, pos = [], 0
-        for i in range(got.value):
-            out.append(raw[pos:pos + rl[i]].decode())
-            pos += rl[i]
-        return out, [ec[i] for i in range(got.value)]
+        return _split_text(raw, rl, got.value), [ec[i] for i in range(got.value)]
 
     def in_flight(self):
         return lib().es_scan_in_flight(self.h)
