@@ -1431,7 +1431,11 @@ struct DeviceCtx::Impl {
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
   Slot slot[DeviceCtx::kMaxInFlight];
   int head = 0;                     // the oldest batch in flight
-  int busy = 0;                     // batches in flight (0 .. kMaxInFlight)
+  int busy = 0;                     // batches in flight or reserved (0 .. kMaxInFlight)
+  // Batches may be launched by one thread while another finishes older ones: slot_mu
+  // guards head and busy; launch_mu serialises whatever enqueues work or reallocates
+  // shared device buffers (launches, overflow replays, layout uploads).
+  std::mutex slot_mu, launch_mu;
 };
 
 namespace {
@@ -1562,6 +1566,7 @@ int DeviceCtx::open(int device) {
 int DeviceCtx::add_layout(const Layout &L) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
+  std::lock_guard<std::mutex> lk(m.launch_mu);
   HIP_TRY(hipSetDevice(device_));
   DevLayoutBufs b;
   auto up = [&](void **dst, const void *src, size_t bytes) -> int {
@@ -1742,7 +1747,24 @@ int DeviceCtx::enqueue_batch(int si) {
 int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
-  if (m.busy == kMaxInFlight) { err_ = "every batch slot is in flight"; return -1; }
+  // Reserve the slot after the newest one in flight; released again if the launch fails.
+  int si;
+  {
+    std::lock_guard<std::mutex> lk(m.slot_mu);
+    if (m.busy == kMaxInFlight) { err_ = "every batch slot is in flight"; return -1; }
+    si = (m.head + m.busy) % kMaxInFlight;
+    ++m.busy;
+  }
+  struct Release {
+    Impl &m;
+    bool keep = false;
+    ~Release() {
+      if (keep) return;
+      std::lock_guard<std::mutex> lk(m.slot_mu);
+      if (--m.busy == 0) m.head = 0;
+    }
+  } release{m};
+  std::lock_guard<std::mutex> launch_lk(m.launch_mu);
   HIP_TRY(hipSetDevice(device_));
   const size_t nt = tiles.size();
   if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
@@ -1759,7 +1781,6 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     }
   }
   if (nt > 0xFFFFFFFFull || nwin > 0x55555555u) { err_ = "batch too large"; return -1; }
-  const int si = (m.head + m.busy) % kMaxInFlight;
   Slot &b = m.slot[si];
   b.nt = nt;
   b.nwin = nwin;
@@ -1821,18 +1842,24 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     b.grid = (u32)((size_t)cus_ * m.occupancy_blocks);
     if (enqueue_batch(si)) return -1;
   }
-  ++m.busy;
+  release.keep = true;
   return 0;
 }
 
 int DeviceCtx::finish(BatchResult &out) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
-  if (m.busy == 0) { err_ = "no batch in flight"; return -1; }
-  const int rc = read_batch(m.head, out);
+  int si;
+  {
+    std::lock_guard<std::mutex> lk(m.slot_mu);
+    if (m.busy == 0) { err_ = "no batch in flight"; return -1; }
+    si = m.head;
+  }
+  const int rc = read_batch(si, out);
   // The slot is released only now, once its mirrors have been read, and whatever the
   // outcome. With nothing left in flight the next batch takes slot 0 again, so
   // one-at-a-time use keeps to one slot and one graph.
+  std::lock_guard<std::mutex> lk(m.slot_mu);
   m.head = (m.head + 1) % kMaxInFlight;
   if (--m.busy == 0) m.head = 0;
   return rc;
@@ -1879,6 +1906,9 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
     if (wait_slot(si)) return -1;
     float sv = 0, mr = 0, rg = 0;
     if (!use_graph) {
+      // Eager launches time their kernels with events, which may complete a little after
+      // the flag is posted.
+      HIP_TRY(hipEventSynchronize(b.ev3));
       HIP_TRY(hipEventElapsedTime(&sv, b.ev1, b.ev2));
       HIP_TRY(hipEventElapsedTime(&mr, b.ev4, b.ev5));
       HIP_TRY(hipEventElapsedTime(&rg, b.ev6, b.ev7));
@@ -1892,6 +1922,7 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
     // Overflow: the counts are complete, the list is not. Grow and replay the batch
     // (sieve/cuda/scan.cu:223-242 does the same).
     ++out.replays;
+    std::lock_guard<std::mutex> launch_lk(m.launch_mu);
     if (grow_surv(si, cnt)) return -1;
     b.h_hdr->seq = ++b.seq;
     if (enqueue_batch(si)) return -1;

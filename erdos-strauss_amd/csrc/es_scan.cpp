@@ -369,14 +369,27 @@ int Scanner::ensure_layout(u64 Mp, std::string &msg) {
 }
 
 int Scanner::device_submit(const std::vector<Interval> &wins, std::string &msg) {
-  const double t0 = now_ms();
   pending_.emplace_back();
   Pending &pb = pending_.back();
   pb.wins = wins;
+  if (plan_and_launch(pb, msg)) {
+    pending_.pop_back();
+    return -1;
+  }
+  pb.launched = true;
+  return 0;
+}
+
+// Plans pb's windows into tiles and launches its device batch. Touches only pb, the
+// layouts and the device context, so the submission thread can run it while the caller
+// collects older batches.
+int Scanner::plan_and_launch(Pending &pb, std::string &msg) {
+  const double t0 = now_ms();
+  const std::vector<Interval> &wins = pb.wins;
   ScanTiming &tm = pb.tm;
   std::vector<WindowRecord> &out = pb.out;
   out.assign(wins.size(), WindowRecord());
-  auto fail = [&]() { pending_.pop_back(); return -1; };
+  auto fail = [&]() { return -1; };
   const u32 maxb = tune_.launch.max_bits;
   // Tiles go out longest first, so the persistent queue ends with the short ones. Only
   // load balance depends on that order (never a result), so tiles are placed straight
@@ -436,9 +449,15 @@ int Scanner::device_submit(const std::vector<Interval> &wins, std::string &msg) 
 
 int Scanner::device_collect(std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
                             ScanTiming &tm, std::string &msg) {
-  if (pending_.empty()) { msg = "FATAL: no batch in flight"; return -1; }
-  Pending pb = std::move(pending_.front());
-  pending_.pop_front();
+  Pending pb;
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (pending_.empty()) { msg = "FATAL: no batch in flight"; return -1; }
+    cv_launched_.wait(lk, [&] { return pending_.front().launched; });   // the submission thread's turn
+    pb = std::move(pending_.front());
+    pending_.pop_front();
+  }
+  if (pb.rc) { msg = pb.msg; return -1; }
   const double t1 = now_ms();
   wins = std::move(pb.wins);
   out = std::move(pb.out);
@@ -478,7 +497,7 @@ int Scanner::device_collect(std::vector<Interval> &wins, std::vector<WindowRecor
 
 int Scanner::device_pass(const std::vector<Interval> &wins, std::vector<WindowRecord> &out, BatchResult &br,
                          ScanTiming &tm, std::string &msg) {
-  if (!pending_.empty()) { msg = "FATAL: batches are in flight: collect them first"; return -1; }
+  if (in_flight()) { msg = "FATAL: batches are in flight: collect them first"; return -1; }
   std::vector<Interval> w;
   if (device_submit(wins, msg)) return -1;
   return device_collect(w, out, br, tm, msg);
@@ -511,8 +530,45 @@ int Scanner::scan(const std::vector<Interval> &wins, std::vector<WindowRecord> &
 }
 
 int Scanner::submit(const std::vector<Interval> &wins, std::string &msg) {
+  std::lock_guard<std::mutex> lk(mu_);
   if (pending_.size() >= (size_t)DeviceCtx::kMaxInFlight) { msg = "FATAL: every batch slot is in flight"; return -1; }
-  return device_submit(wins, msg);
+  if (!worker_.joinable()) worker_ = std::thread([this] { submission_loop(); });
+  pending_.emplace_back();
+  pending_.back().wins = wins;
+  ++unlaunched_;
+  cv_work_.notify_one();
+  return 0;
+}
+
+// The submission thread: plans and launches submitted batches in order, so the caller's
+// collect (records of batch i) runs while batches i+1.. are being planned and launched.
+// Only this thread launches while batches are in flight; the caller then waits on the
+// batches' pinned completion flags and makes no HIP call meanwhile.
+void Scanner::submission_loop() {
+  std::unique_lock<std::mutex> lk(mu_);
+  for (;;) {
+    cv_work_.wait(lk, [&] { return stop_ || unlaunched_ > 0; });
+    if (stop_ && unlaunched_ == 0) return;
+    Pending &pb = pending_[pending_.size() - unlaunched_];   // references survive push_back/pop_front
+    lk.unlock();
+    std::string msg;
+    const int rc = plan_and_launch(pb, msg);
+    lk.lock();
+    pb.rc = rc;
+    pb.msg = msg;
+    pb.launched = true;
+    --unlaunched_;
+    cv_launched_.notify_all();
+  }
+}
+
+Scanner::~Scanner() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_one();
+  if (worker_.joinable()) worker_.join();
 }
 
 int Scanner::collect(std::vector<WindowRecord> &out, ScanTiming &tm, std::string &msg) {

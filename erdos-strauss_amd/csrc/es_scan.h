@@ -2,7 +2,10 @@
 // layout per window, cut windows into tiles, run the device batch, certify the primes
 // on the host, and assemble each window's record exactly as rung_scan3 prints it.
 #pragma once
+#include <condition_variable>
 #include <deque>
+#include <mutex>
+#include <thread>
 #include <functional>
 #include <map>
 #include <memory>
@@ -52,6 +55,10 @@ struct Tuning {
 
 class Scanner {
  public:
+  Scanner() = default;
+  ~Scanner();
+  Scanner(const Scanner &) = delete;
+  Scanner &operator=(const Scanner &) = delete;
   // Phase 1: wheel + pin (and default tuning). Returns 0 or 1 with msg.
   int init_wheel(u64 M, u64 mmax, std::string &msg);
   // Phase 2: filter set. Returns 0 or 1 with msg.
@@ -70,7 +77,10 @@ class Scanner {
   // in flight.
   int submit(const std::vector<Interval> &wins, std::string &msg);
   int collect(std::vector<WindowRecord> &out, ScanTiming &tm, std::string &msg);
-  size_t in_flight() const { return pending_.size(); }
+  size_t in_flight() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return pending_.size();
+  }
   // Device stages only: each window's counters (positions..composite) and all stage-C
   // survivors of the batch, ascending by (window, p), with their primality.
   int survivors(const std::vector<Interval> &wins, std::vector<WindowRecord> &out,
@@ -129,8 +139,20 @@ class Scanner {
     std::vector<Interval> wins;
     std::vector<WindowRecord> out;   // lo, hi and layout filled at submission
     ScanTiming tm;                   // plan and launch times so far
+    bool launched = false;           // planned and launched (or failed: rc, msg)
+    int rc = 0;
+    std::string msg;
   };
+  int plan_and_launch(Pending &pb, std::string &msg);
+  void submission_loop();
+  // Batches submitted and not collected, oldest first. The submission thread launches
+  // the last unlaunched_ of them; mu_ guards the deque and the counters.
   std::deque<Pending> pending_;
+  mutable std::mutex mu_;
+  std::condition_variable cv_work_, cv_launched_;
+  std::thread worker_;
+  size_t unlaunched_ = 0;
+  bool stop_ = false;
   Plan plan_;
   Tuning tune_;
   DeviceCtx dev_;
