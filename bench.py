@@ -36,7 +36,7 @@ Also reported, in the same JSON line:
                and work separated; cpu_baseline_port: the oracle over all 512 windows
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-legs]
-                       [--no-tuned] [--c5-seconds S]
+                       [--no-tuned] [--c5-seconds S] [--calib-copy MIB]
 """
 import argparse
 import hashlib
@@ -419,11 +419,20 @@ def main():
     ap.add_argument("--no-tuned", action="store_true", help="skip the MI355X-tuned filter table leg")
     ap.add_argument("--c5-seconds", type=float, default=60.0,
                     help="seconds of the serving trace to replay in real time (0 = skip)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a kernel of known size for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve shards into independent intervals with no "
               "exchange step, and only the one-GPU configuration is built here.", file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        import torch
+        src = torch.ones(args.calib_copy << 18, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     es = load("es_sieve")
     records_mod = load("records")
 

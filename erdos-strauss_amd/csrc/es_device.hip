@@ -1269,8 +1269,8 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
 // the survivor count, the per-tile counters, and per survivor p, window, primality flag
 // and (primes only) the stage-D result.
 // The last of its workgroups to finish posts the batch's end time and number into pinned
-// host memory (h_sync[2], h_sync[0]) after system-scope fences, so the host sees the
-// batch complete, with every mirror written, by reading host memory alone.
+// host memory (h_sync[2], h_sync[0]) once every mirror store has completed, so the host
+// sees the batch complete, with every mirror written, by reading host memory alone.
 __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict__ tile_counts,
                        const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
                        const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
@@ -1288,12 +1288,13 @@ __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict
     h_f[i] = f;
     if (with_rung && f) h_r[i] = rung[i];
   }
-  __threadfence_system();
+  // Every wave waits until its own stores to host memory have completed (a fence at system
+  // scope would also write back the whole L2, once per wave); the last workgroup to count
+  // itself done then posts the end time and, with one system-scope release, the number.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0 && atomicAdd(done_blocks, 1u) == gridDim.x - 1) {
-    __threadfence_system();
     h_sync[2] = __builtin_amdgcn_s_memrealtime();
-    __threadfence_system();
     __hip_atomic_store(&h_sync[0], (u64)hdr->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
