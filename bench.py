@@ -136,17 +136,27 @@ def one_at_a_time(sc, batch, steps, want):
     return time.perf_counter() - t0, lat, ok
 
 
-def eager_split(sc, batch, want, reps=5):
-    """Kernel times from HIP events on each dispatch (eager launches, graph off)."""
-    sc.set_graph(False)
+def kernel_split(sc, batch, want, reps=5):
+    """Kernel times of one batch: from the kernels' own device-clock stamps over `reps`
+    graph-replayed passes, one at a time (the headline form), and from HIP events over
+    `reps` eager passes (each eager dispatch that carries events also waits out a marker
+    before it, which the events include)."""
     rows, ok = [], True
     for _ in range(reps):
         recs, _ = sc.records(batch)
         ok = ok and recs == want
         rows.append(sc.timing())
+    sc.set_graph(False)
+    erows = []
+    for _ in range(reps):
+        recs, _ = sc.records(batch)
+        ok = ok and recs == want
+        erows.append(sc.timing())
     sc.set_graph(True)
     keys = ("device_ms", "sieve_ms", "mr_ms", "rung_ms", "tail_ms", "plan_ms")
     out = {k: med([r[k] for r in rows]) for k in keys}
+    out["eager"] = {k: med([r[k] for r in erows])
+                    for k in ("sieve_ms", "mr_ms", "rung_ms", "sieve_event_ms", "mr_event_ms", "rung_event_ms")}
     out.update(tiles=rows[-1]["tiles"], survivors=rows[-1]["survivors"], primes=rows[-1]["primes"],
                host_primes=rows[-1]["host_primes"])
     return out, ok
@@ -167,6 +177,8 @@ def marking_roofline(sc, batch, sieve_ms, layouts=None):
     return {"bound": "lds", "kernel": "k_sieve", "achieved": achieved, "peak": NOMINAL_LDS_BPS / 1e9,
             "unit": "GB/s", "frac": achieved / (NOMINAL_LDS_BPS / 1e9), "algorithmic_bytes": lds_bytes,
             "marks": marks, "bits": bits, "pairs": pairs, "tiles": tiles, "kernel_ms": sieve_ms,
+            "kernel_ms_from": "the kernel's own device-clock stamps (first workgroup's start to last "
+                              "workgroup's end), median of 5 graph replays one at a time",
             "marks_per_s": marks / (sieve_ms * 1e-3)}
 
 
@@ -185,8 +197,9 @@ def c4_wheel(es, M, F, steps, warmup):
     seq_wall, lat, ok1 = one_at_a_time(sc, wins, steps, gold)
     pipelined(sc, wins, es.MAX_IN_FLIGHT, gold)                 # every slot's graph captured
     wall, ok2 = pipelined(sc, wins, steps, gold)
-    split, ok3 = eager_split(sc, wins, gold)
+    split, ok3 = kernel_split(sc, wins, gold)
     roof = marking_roofline(sc, wins, split["sieve_ms"])
+    roof["kernel_ms_hip_events_eager"] = split["eager"]["sieve_event_ms"]
     sc.set_graph(False)
     sc.set_launch(False, block, bits)
     t = time.perf_counter()
@@ -221,8 +234,9 @@ def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
     seq_wall, lat, ok1 = one_at_a_time(sc, batch, steps, want)
     pipelined(sc, batch, es.MAX_IN_FLIGHT, want)
     wall, ok2 = pipelined(sc, batch, steps, want)
-    split, ok3 = eager_split(sc, batch, want)
+    split, ok3 = kernel_split(sc, batch, want)
     roof = marking_roofline(sc, batch, split["sieve_ms"])
+    roof["kernel_ms_hip_events_eager"] = split["eager"]["sieve_event_ms"]
     out = {"interval": [lo, hi], "wheel": M, "table": os.path.relpath(F, REPO),
            "value": (hi - lo) * steps / wall, "ms_per_step": wall * 1e3 / steps,
            "one_at_a_time": {"value": (hi - lo) * steps / seq_wall, "ms_per_step": seq_wall * 1e3 / steps},
@@ -343,7 +357,7 @@ def host_info():
     return model, smt
 
 
-def cpu_baseline(cores, every=4, hi_cap=10 ** 13):
+def cpu_baseline(cores, every=2, hi_cap=10 ** 13):
     """The reference scanner (oracle/_ref/rung_scan3, built from the reference's own
     sources) on every `every`-th batched window with hi < hi_cap, one process per window as
     rung_scan3 runs, --threads cores. Its startup (a base-prime sieve to sqrt(hi), which

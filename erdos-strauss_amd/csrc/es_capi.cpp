@@ -298,7 +298,8 @@ int es_last_timing(const es_scanner *s, es_timing *t) {
   if (!s || !t) return fail(ES_ERR_ARG, "null argument");
   const es::ScanTiming &m = s->last;
   *t = es_timing{m.plan_ms, m.batch_ms, m.device_ms, m.sieve_ms, m.tail_ms, m.total_ms,
-                 m.tiles, m.survivors, m.primes, m.host_primes, m.replays, m.mr_ms, m.rung_ms};
+                 m.tiles, m.survivors, m.primes, m.host_primes, m.replays, m.mr_ms, m.rung_ms,
+                 m.sieve_event_ms, m.mr_event_ms, m.rung_event_ms};
   return ES_OK;
 }
 

@@ -43,9 +43,14 @@ struct BatchResult {
   std::vector<uint8_t> surv_prime;    // 1 if prime (deterministic MR on the device)
   std::vector<RungResult> rung;       // per survivor, when stage D ran on the device
   double device_ms = 0;               // sieve + MR kernels, HIP events
+  // Kernel times from the kernels' own device-clock stamps (first workgroup's start to
+  // last wave's end), graph replays included.
   double sieve_ms = 0;                // sieve kernel alone
   double mr_ms = 0;                   // base-2 Miller-Rabin kernel alone
   double tail_kernel_ms = 0;          // stage D kernel (or the other MR bases) alone
+  // The same three from HIP events on the eager dispatches (-1 from a graph replay): an
+  // eager dispatch that carries events also carries the marker gap before it.
+  double sieve_event_ms = 0, mr_event_ms = 0, tail_event_ms = 0;
   int replays = 0;                    // survivor-buffer overflows that forced a rerun
 };
 

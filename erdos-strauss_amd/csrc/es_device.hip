@@ -62,6 +62,7 @@ struct LayoutDev {
 // Per-batch scalars: survivor count, tile queue, stage-D queue, survivors stage D has
 // finished, workgroups of k_pack done.
 constexpr u32 kScalars = 5;
+constexpr u32 kStamps = 6;   // (start, end) of k_sieve, k_mr_base2, k_rung / k_mr_bases
 
 struct BatchHdr {
   u32 ntiles, nwin, has_sieve, seq;   // seq: the batch's number, posted back when it completes
@@ -182,6 +183,17 @@ __device__ __forceinline__ u32 wave_sum(u32 v) {
 // Each tile writes its counters {positions, alive after covers, survivors} to its own
 // slot of tile_counts: no counter is shared between workgroups (the host sums them per
 // window), so only tiles with survivors touch a shared word (the survivor count).
+// Kernel times on the device's constant clock: the first workgroup to start lowers
+// stamp[0], the last wave (or workgroup) to finish raises stamp[1]. The host reads the
+// pair back with the batch (k_pack copies it), for eager launches and graph replays alike.
+// k_stage sets stamp[0] to ~0 and stamp[1] to 0 before every batch; null skips the stamps.
+__device__ __forceinline__ void stamp_start(u64 *stamp) {
+  if (stamp && threadIdx.x == 0) atomicMin((unsigned long long *)stamp, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+__device__ __forceinline__ void stamp_end(u64 *stamp, bool writer) {
+  if (stamp && writer) atomicMax((unsigned long long *)stamp + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
 template <bool kPersistent, bool kRowInLds>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
@@ -189,7 +201,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
                                                 uint4 *__restrict__ tile_counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
                                                 u32 *__restrict__ surv_count, u32 *__restrict__ queue,
-                                                u32 lds_words, u32 off_words) {
+                                                u32 lds_words, u32 off_words, u64 *stamp) {
   extern __shared__ u32 lds[];
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
@@ -201,6 +213,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   const u32 lane64 = tid & 63u, wave = tid >> 6, nwaves = (bs + 63u) >> 6;
   const u32 ntiles = hdr->ntiles;
   const bool sieve = hdr->has_sieve != 0;
+  stamp_start(stamp);
   u32 t = blockIdx.x;
   for (;;) {
     if (kPersistent) {
@@ -307,6 +320,9 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     if (!kPersistent) break;
     __syncthreads();   // the next tile reuses the map and the scalars
   }
+  // Every wave leaves the loop at the same barrier (persistent) or after the tile's last
+  // barrier, so thread 0's time is the workgroup's.
+  stamp_end(stamp, threadIdx.x == 0);
 }
 
 // Stages one batch: zeroes the survivor count and the queue heads, and copies the batch
@@ -318,13 +334,15 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
 // capacity. Every other per-batch output is written in full by the kernels.
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
                         BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars,
-                        u64 *__restrict__ board_words, u32 nboard_words, u64 *__restrict__ h_sync) {
+                        u64 *__restrict__ board_words, u32 nboard_words, u64 *__restrict__ h_sync,
+                        u64 *__restrict__ stamps) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const BatchHdr h = *h_hdr;
   if (g == 0) {
     *hdr = h;
     h_sync[1] = __builtin_amdgcn_s_memrealtime();   // the batch's start on the device clock
   }
+  if (g < kStamps) stamps[g] = (g & 1) ? 0ull : ~0ull;   // kernel (start, end) pairs
   if (g < kScalars) scalars[g] = 0;   // survivor count, tile queue, stage-D queue, survivors finished,
                                       // workgroups of k_pack done
   for (u32 i = g; i < nboard_words; i += stride) board_words[i] = 0;   // stage D's board of posted splits
@@ -422,7 +440,8 @@ constexpr u8 kComposite = 0, kPrime = 1, kPending = 2;
 // Pass 1, one thread per value: the trial prologue by the first twelve primes and the
 // base-2 test. Values that pass are marked pending and keep 2^64 mod n for pass 2.
 __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
-                           u8 *__restrict__ flags, u64 *__restrict__ one_out) {
+                           u8 *__restrict__ flags, u64 *__restrict__ one_out, u64 *stamp = nullptr) {
+  stamp_start(stamp);
   const u32 n_ = min(*count, cap);
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n_; i += gridDim.x * blockDim.x) {
     const u64 n = vals[i];
@@ -455,13 +474,18 @@ __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__
     }
     flags[i] = v;
   }
+  if (stamp) {
+    __syncthreads();
+    stamp_end(stamp, threadIdx.x == 0);
+  }
 }
 
 // Pass 2, one thread per (value, base) for the eleven odd prime bases: any failing
 // base writes the composite verdict (all writers write the same byte value). Pending
 // values that survive every base are prime.
 __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
-                           u8 *__restrict__ flags, const u64 *__restrict__ one_in) {
+                           u8 *__restrict__ flags, const u64 *__restrict__ one_in, u64 *stamp = nullptr) {
+  stamp_start(stamp);
   const u32 n_ = min(*count, cap);
   const u32 total = n_ * 11u;
   for (u32 g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
@@ -481,6 +505,10 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
       ok = strong_probable(n, ninv, one_in[i], c_small_p[b], d, s);
     }
     if (!ok) flags[i] = kComposite;
+  }
+  if (stamp) {
+    __syncthreads();
+    stamp_end(stamp, threadIdx.x == 0);
   }
 }
 
@@ -1089,7 +1117,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
                                                           TrialTab T, RungOut *__restrict__ out, u32 *queue,
-                                                          RhoBoard *board, u32 *done) {
+                                                          RhoBoard *board, u32 *done, u64 *stamp = nullptr) {
   __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -1103,6 +1131,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
+  stamp_start(stamp);
   // A persistent grid takes survivors from a queue, so every prime starts as soon as a
   // workgroup is free (the index slot alternates, so the next claim never overwrites an
   // index a wave has yet to read).
@@ -1262,6 +1291,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     }
     __syncthreads();
   }
+  stamp_end(stamp, lane == 0);   // per wave: helper waves leave later than the others
 }
 
 // Copies one batch's results into pinned host memory with plain stores over the bus
@@ -1271,15 +1301,17 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
 // The last of its workgroups to finish posts the batch's end time and number into pinned
 // host memory (h_sync[2], h_sync[0]) once every mirror store has completed, so the host
 // sees the batch complete, with every mirror written, by reading host memory alone.
+// h_sync[3 ..) receives the kernels' device-clock stamps.
 __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict__ tile_counts,
                        const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
                        const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
                        const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, uint4 *h_tc,
                        u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r, u32 *__restrict__ done_blocks,
-                       u64 *h_sync) {
+                       u64 *h_sync, const u64 *__restrict__ stamps) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const u32 nt = hdr->ntiles, n = min(*surv_count, cap);
   if (g == 0) h_count[0] = *surv_count;
+  if (g < kStamps) h_sync[3 + g] = stamps[g];
   for (u32 i = g; i < nt; i += stride) h_tc[i] = tile_counts[i];
   for (u32 i = g; i < n; i += stride) {
     const u8 f = flags[i];
@@ -1362,6 +1394,8 @@ struct DevLayoutBufs {
 // behind the host. Four slots, one stream each, match the four hardware queues HIP gives
 // a process by default; with more streams than queues, streams share a queue and their
 // batches serialise (measured: six slots ran C2 steps at 90 us against four slots' 80).
+constexpr size_t kSyncWords = 3 + dev::kStamps;
+
 struct Slot {
   // Each slot has its own stream, so the device may also overlap batches: one batch's
   // stage D (a few primes on a few CUs) with the next batch's staging and sieve.
@@ -1392,7 +1426,9 @@ struct Slot {
   // Pinned host memory: tile descriptors staged for the upload, and the mirrors k_pack
   // writes the results into (device-side addresses in the dh_* twins).
   u32 *h_count = nullptr;
-  u64 *h_sync = nullptr;            // [0] number of the last batch completed, [1] its start, [2] its end
+  u64 *h_sync = nullptr;            // [0] number of the last batch completed, [1] its start, [2] its end,
+                                    // [3, 3 + kStamps) its kernels' (start, end) stamps
+  u64 *d_stamps = nullptr;
   u64 *dh_sync = nullptr;
   u32 seq = 0;                      // number of the batch in flight in this slot
   TileDesc *h_tiles = nullptr;
@@ -1462,7 +1498,7 @@ DeviceCtx::~DeviceCtx() {
     for (Slot &b : m.slot) {
       for (void *p : {(void *)b.d_tiles, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
                       (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung,
-                      (void *)b.d_scalars, (void *)b.d_board})
+                      (void *)b.d_scalars, (void *)b.d_board, (void *)b.d_stamps})
         if (p) (void)hipFree(p);
       for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_sync, (void *)b.h_tiles, (void *)b.h_tile_counts,
                       (void *)b.h_surv_p, (void *)b.h_surv_w, (void *)b.h_flags, (void *)b.h_rung})
@@ -1530,9 +1566,10 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
     HIP_TRY(hipMalloc(&b.d_scalars, dev::kScalars * sizeof(u32)));
-    HIP_TRY(hipHostMalloc(&b.h_sync, 4 * sizeof(u64), hipHostMallocMapped));
+    HIP_TRY(hipMalloc(&b.d_stamps, dev::kStamps * sizeof(u64)));
+    HIP_TRY(hipHostMalloc(&b.h_sync, kSyncWords * sizeof(u64), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_sync, b.h_sync, 0));
-    std::memset(b.h_sync, 0, 4 * sizeof(u64));
+    std::memset(b.h_sync, 0, kSyncWords * sizeof(u64));
     HIP_TRY(hipMalloc(&b.d_board, dev::kBoardSlots * sizeof(dev::RhoBoard)));
     HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
@@ -1674,7 +1711,7 @@ int DeviceCtx::enqueue_batch(int si) {
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
                                                         (uint2 *)b.d_tiles, b.d_scalars, (u64 *)b.d_board,
-                                                        dev::kBoardSlots * 4u, b.dh_sync);
+                                                        dev::kBoardSlots * 4u, b.dh_sync, b.d_stamps);
     HIP_TRY(hipGetLastError());
     // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
     // not by separate event packets around it.
@@ -1683,35 +1720,35 @@ int DeviceCtx::enqueue_batch(int si) {
       hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<true, true> : dev::k_sieve<true, false>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
-                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, b.d_stamps);
     } else {
       hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<false, true> : dev::k_sieve<false, false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
-                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words);
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, b.d_stamps);
     }
     HIP_TRY(hipGetLastError());
     hipExtLaunchKernelGGL(dev::k_mr_base2, dim3(std::max(1, cus_ * 2)), dim3(256), 0, st, timed ? b.ev4 : nullptr,
                           timed ? b.ev5 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
-                          (u32)b.surv_cap, b.d_flags, b.d_one);
+                          (u32)b.surv_cap, b.d_flags, b.d_one, b.d_stamps + 2);
     HIP_TRY(hipGetLastError());
     if (cfg.device_tail) {
       // k_rung finishes the Miller-Rabin bases of the pending values itself.
       hipExtLaunchKernelGGL(dev::k_rung, dim3(std::max(1, cus_)), dim3(64 * dev::kRungWaves), 0, st,
                             timed ? b.ev6 : nullptr, timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p,
                             (const u32 *)b.d_scalars, (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, m.trial,
-                            b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3);
+                            b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3, b.d_stamps + 4);
     } else {
       hipExtLaunchKernelGGL(dev::k_mr_bases, dim3(std::max(1, cus_ * 4)), dim3(256), 0, st, timed ? b.ev6 : nullptr,
                             timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
-                            (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one);
+                            (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, b.d_stamps + 4);
     }
     HIP_TRY(hipGetLastError());
     dev::k_pack<<<std::max(1, cus_), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
                                                    b.d_surv_p, b.d_surv_w, b.d_flags, b.d_rung,
                                                    cfg.device_tail ? 1u : 0u, b.dh_count, b.dh_tile_counts,
                                                    b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung, b.d_scalars + 4,
-                                                   b.dh_sync);
+                                                   b.dh_sync, (const u64 *)b.d_stamps);
     HIP_TRY(hipGetLastError());
     return 0;
   };
@@ -1901,6 +1938,7 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
   out.rung.clear();
   out.replays = 0;
   out.device_ms = out.sieve_ms = out.mr_ms = out.tail_kernel_ms = 0;
+  out.sieve_event_ms = out.mr_event_ms = out.tail_event_ms = 0;
   if (b.nt == 0) return 0;
   const bool use_graph = b.cfg.graph && b.cfg.persistent;
   for (;;) {
@@ -1914,10 +1952,15 @@ int DeviceCtx::read_batch(int si, BatchResult &out) {
       HIP_TRY(hipEventElapsedTime(&mr, b.ev4, b.ev5));
       HIP_TRY(hipEventElapsedTime(&rg, b.ev6, b.ev7));
     }
+    const u64 *st = b.h_sync + 3;
+    auto span = [&](int k) { return st[2 * k + 1] > st[2 * k] ? (double)(st[2 * k + 1] - st[2 * k]) * clock_ms_per_tick_ : 0.0; };
     out.device_ms += (double)(b.h_sync[2] - b.h_sync[1]) * clock_ms_per_tick_;
-    out.sieve_ms += use_graph ? -1.0 : sv;
-    out.mr_ms += use_graph ? -1.0 : mr;
-    out.tail_kernel_ms += use_graph ? -1.0 : rg;
+    out.sieve_ms += span(0);
+    out.mr_ms += span(1);
+    out.tail_kernel_ms += span(2);
+    out.sieve_event_ms += use_graph ? -1.0 : sv;
+    out.mr_event_ms += use_graph ? -1.0 : mr;
+    out.tail_event_ms += use_graph ? -1.0 : rg;
     const u32 cnt = *b.h_count;
     if (cnt <= b.surv_cap) break;
     // Overflow: the counts are complete, the list is not. Grow and replay the batch

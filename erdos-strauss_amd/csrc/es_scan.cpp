@@ -472,6 +472,9 @@ int Scanner::device_collect(std::vector<Interval> &wins, std::vector<WindowRecor
   tm.sieve_ms = br.sieve_ms;
   tm.mr_ms = br.mr_ms;
   tm.rung_ms = br.tail_kernel_ms;
+  tm.sieve_event_ms = br.sieve_event_ms;
+  tm.mr_event_ms = br.mr_event_ms;
+  tm.rung_event_ms = br.tail_event_ms;
   tm.replays = br.replays;
   tm.survivors = br.surv_p.size();
 

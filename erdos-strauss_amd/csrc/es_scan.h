@@ -30,7 +30,9 @@ struct WindowRecord {
 
 struct ScanTiming {
   double plan_ms = 0, device_ms = 0, sieve_ms = 0, batch_ms = 0, tail_ms = 0, total_ms = 0;
-  double mr_ms = 0, rung_ms = 0;   // base-2 MR kernel, stage D kernel (eager launches only; -1 from a graph)
+  double mr_ms = 0, rung_ms = 0;   // base-2 MR kernel, stage D kernel (device-clock stamps, like sieve_ms)
+  // The three kernel times from HIP events (eager launches only; -1 from a graph replay).
+  double sieve_event_ms = 0, mr_event_ms = 0, rung_event_ms = 0;
   u64 tiles = 0, primes = 0, survivors = 0;
   u64 host_primes = 0;        // primes whose stage D/E ran on the host
   int replays = 0;

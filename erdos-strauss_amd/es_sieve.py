@@ -37,7 +37,9 @@ class EsTiming(ctypes.Structure):
     _fields_ = [("plan_ms", ctypes.c_double), ("batch_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
                 ("sieve_ms", ctypes.c_double), ("tail_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("tiles", U64), ("survivors", U64), ("primes", U64), ("host_primes", U64),
-                ("replays", ctypes.c_int), ("mr_ms", ctypes.c_double), ("rung_ms", ctypes.c_double)]
+                ("replays", ctypes.c_int), ("mr_ms", ctypes.c_double), ("rung_ms", ctypes.c_double),
+                ("sieve_event_ms", ctypes.c_double), ("mr_event_ms", ctypes.c_double),
+                ("rung_event_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

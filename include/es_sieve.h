@@ -55,6 +55,10 @@ typedef struct es_timing {
   uint64_t tiles, survivors, primes, host_primes;
   int replays;
   double mr_ms, rung_ms;
+  /* The kernel times above (sieve_ms, mr_ms, rung_ms) come from the kernels' own stamps on
+   * the device's constant clock, graph replays included; these three are the same times
+   * from HIP events on eager launches (-1 when the batch was replayed from a graph). */
+  double sieve_event_ms, mr_event_ms, rung_event_ms;
 } es_timing;
 
 /* Fills *cfg with rung_scan3's defaults. */
