@@ -213,9 +213,35 @@ int es_certify_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, si
   for (size_t i = 0; i < n; ++i) {
     if (rr[i].status == 1) {
       es::format_rung(p[i], rr[i].A, rr[i].r, ((es::u128)rr[i].u_hi << 64) | rr[i].u_lo, line);
+    } else if (rr[i].status == 4) {
+      es::format_direct(p[i], rr[i].r & 0xFFu, rr[i].A, rr[i].r >> 8, rr[i].u_lo, line);
     } else {
       ++host;
       es::certify_prime(p[i], line);
+    }
+    all += line;
+  }
+  *text = (char *)malloc(all.size() + 1);
+  if (!*text) return fail(ES_ERR_ARG, "out of host memory");
+  memcpy(*text, all.c_str(), all.size() + 1);
+  if (on_host) *on_host = host;
+  return ES_OK;
+}
+
+int es_direct_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, size_t *on_host) {
+  if (!s || !text || (n && !p)) return fail(ES_ERR_ARG, "null argument");
+  std::vector<es::RungResult> rr(n);
+  if (s->sc.device().rung(p, n, rr.data(), true)) return fail(ES_ERR_DEVICE, s->sc.device().last_error());
+  std::string all, line;
+  size_t host = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (rr[i].status == 4) {
+      es::format_direct(p[i], rr[i].r & 0xFFu, rr[i].A, rr[i].r >> 8, rr[i].u_lo, line);
+    } else if (rr[i].status == 2) {
+      line = "SURVIVOR " + std::to_string(p[i]) + "\n";
+    } else {
+      ++host;
+      if (!es::solve_direct(p[i], line)) line = "SURVIVOR " + std::to_string(p[i]) + "\n";
     }
     all += line;
   }

@@ -98,8 +98,11 @@ class DeviceCtx {
   // without allocating (state in the library's fixed device scratch). -2 for an argument
   // error (too many values, not device memory), -1 for a HIP failure.
   int primality_noalloc(const u64 *d_vals, size_t n, uint8_t *d_flags, void *stream);
-  // Stage D on the device for primes vals[0..n) (one RungResult each).
-  int rung(const u64 *vals, size_t n, RungResult *out);
+  // Stage D on the device for primes vals[0..n) (one RungResult each), then stage E for
+  // those no rung certifies; direct_only skips stage D (every prime goes to stage E).
+  // status: 1 RUNG (A, r, u), 2 neither stage certified p, 3 declined (the host
+  // finishes p), 4 SOLVED (A = c, u_lo = k, r = a | d << 8).
+  int rung(const u64 *vals, size_t n, RungResult *out, bool direct_only = false);
   // Measured LDS atomic-AND rate of the whole device (32-bit operations per second).
   int lds_peak(double &ops_per_s);
   int lds_scatter_peak(double &stores_per_s);   // scattered byte stores per second

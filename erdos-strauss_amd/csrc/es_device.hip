@@ -552,6 +552,10 @@ struct RungOut {
   u32 r, status;
 };
 constexpr u32 kRungFound = 1, kRungNone = 2, kRungHost = 3;
+// Stage E's result in a RungOut: A = c, u_lo = k, r = a | d << 8 ("SOLVED p a c d k").
+constexpr u32 kRungDirect = 4;
+constexpr u32 kDirectPairs = 64;              // (a, d) in 1..8 x 1..8, a major
+constexpr u64 kDirectNMaxDev = 18446744065119617025ull;   // (2^32 - 1)^2, as the host's
 constexpr u32 kRungWaves = 8;     // waves per prime = rungs in flight (two per SIMD)
 constexpr u32 kRungSlots = 64;    // candidate rungs r0 + 4j <= 255
 constexpr u32 kSmallFac = 16;     // odd primes < 2^16 dividing one A < 2^62: at most 15
@@ -926,10 +930,12 @@ __device__ __forceinline__ void fac_add(WaveFac &F, u64 q, u32 e, u32 lane) {
   wave_sync();
 }
 
+// floor(sqrt(m)) for every u64 m: the estimate is clamped to 2^32 - 1 first, so no
+// square below overflows (stage E's cofactors reach 2^64).
 __device__ __forceinline__ u64 isqrt_dev(u64 m) {
-  u64 s = (u64)sqrt((double)m);
+  u64 s = min((u64)sqrt((double)m), 0xFFFFFFFFull);
   while (s && s * s > m) --s;
-  while ((s + 1) * (s + 1) <= m) ++s;
+  while (s < 0xFFFFFFFFull && (s + 1) * (s + 1) <= m) ++s;
   return s;
 }
 
@@ -1113,11 +1119,69 @@ __device__ void board_help(RhoBoard *board, const u32 *done, u32 ndone, u32 lane
   }
 }
 
+// ------------------------------------------------------------------ stage E on the device
+//
+// Stage E (rung_scan3.cpp:607-647) for a prime no rung certifies: the first pair (a, d),
+// a then d ascending in 1..8, for which n = p + 4 a^2 d has a divisor m == -1 (mod 4ad)
+// with c = (m + 1) / 4ad, k = n / m and c k > a, and for that pair the least such m.
+// Waves take pairs in order from an LDS counter, as they take rungs; each factors its n
+// completely (trial division by the odd primes below 2^16 across the lanes, then the
+// cofactor as in stage D) and searches the divisors across the lanes, keeping the
+// least m that qualifies, which is the first one in ascending order.
+
+// Indices (into T) of the odd primes below 2^16 dividing n, into small[0 .. min(count, 16));
+// returns the count. Exact divisibility: q | n iff n q^-1 mod 2^64 <= floor((2^64-1)/q).
+__device__ u32 wave_trial(u64 n, const TrialTab &T, u16 *small, u32 lane) {
+  u32 ns = 0;
+  for (u32 i0 = 0; i0 < T.n; i0 += 64) {
+    const u32 i = i0 + lane;
+    const bool hit = i < T.n && n * T.inv[i] <= T.lim[i];
+    const u64 b = __ballot(hit);
+    if (hit) {
+      const u32 slot = ns + (u32)__popcll(b & ((1ull << lane) - 1ull));
+      if (slot < kSmallFac) small[slot] = (u16)i;
+    }
+    ns += (u32)__popcll(b);
+  }
+  wave_sync();
+  return ns;
+}
+
+// The least divisor m of n (factored in F) with m == q4 - 1 (mod q4) and (m+1)/q4 * n/m > a,
+// divisors taken across the lanes in mixed radix; false if none qualifies.
+__device__ bool wave_direct_divisor(u64 n, u64 q4, u64 a, const WaveFac &F, u32 lane, u64 &c, u64 &k) {
+  const u32 nf = F.k;
+  u64 D = 1;
+  for (u32 i = 0; i < nf; ++i) D *= (u64)(F.e[i] + 1);
+  u64 best = ~0ull;
+  for (u64 t = lane; t < D; t += 64) {
+    u64 m = 1, x = t;
+    for (u32 i = 0; i < nf; ++i) {
+      const u64 r = (u64)(F.e[i] + 1);
+      for (u64 e = x % r; e; --e) m *= F.p[i];
+      x /= r;
+    }
+    if (m < best && m % q4 == q4 - 1) {
+      const u64 cc = (m + 1) / q4, kk = n / m;
+      if (cc && kk && (u128)cc * kk > a) best = m;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const u64 other = (u64)__shfl_xor((unsigned long long)best, o, 64);
+    best = min(best, other);
+  }
+  if (best == ~0ull) return false;
+  c = (best + 1) / q4;
+  k = n / best;
+  return true;
+}
+
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
                                                           TrialTab T, RungOut *__restrict__ out, u32 *queue,
-                                                          RhoBoard *board, u32 *done, u64 *stamp = nullptr) {
+                                                          RhoBoard *board, u32 *done, u64 *stamp = nullptr,
+                                                          u32 rung_cap = kRungSlots) {
   __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -1162,7 +1226,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
     const u64 p = vals[idx];
     u32 r0 = (u32)((4 - (p & 3)) & 3);
     if (r0 == 0) r0 = 4;
-    const u32 nj = (255 - r0) / 4 + 1;
+    const u32 nj = min((255 - r0) / 4 + 1, rung_cap);   // rung_cap 0: stage E alone
     const u64 A0 = (p + r0) / 4;
     if (tid < kRungSlots) nfac[tid] = 0;
     if (tid < kRungWaves) wres[tid] = RungOut{0, 0, 0, 0, kRungNone};
@@ -1239,50 +1303,97 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       __syncthreads();
       continue;
     }
-    // Rungs in order. Between rungs a wave first helps a split in progress: that split
-    // belongs to a smaller rung than any it could take, so it is on the critical path.
+    // Rungs in order (phase 0). Between rungs a wave first helps a split in progress: that
+    // split belongs to a smaller rung than any it could take, so it is on the critical path.
+    // A prime no rung certifies (and none declined) goes on to stage E in the same
+    // workgroup (phase 1): the pairs (a, d) in order take the rungs' place, through the
+    // same factorization code.
     u32 seen = 0;
-    for (;;) {
-      help_split(&rs, seen, wv, lane);
-      u32 k = 0;
-      if (lane == 0) k = atomicAdd(&next_k, 1u);
-      k = __shfl(k, 0, 64);
-      if (k >= nj || k > *stop) break;
-      const u32 r = r0 + 4 * k;
-      const u64 A = A0 + k;
-      const u32 mg = r == 1 ? 0xFFFFFFFFu : (u32)(0x100000000ull / r), c32 = (u32)(0x100000000ull % r);
-      const u32 qm = (small_mod(p, r, mg, c32) * small_mod(A, r, mg, c32)) % r;
-      u32 ga = qm, gb = r;
-      while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
-      if (r > 1 && ga != 1) continue;
-      const u32 ns = nfac[k];
-      const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, fac[k], ns, T, F[wv], lane, stop, k, &rs, board);
-      if (fr == kFacStop) break;
-      if (fr == kFacHost) {
-        if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
-        break;
-      }
-      const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], lane);
-      if (u != ~(u128)0) {
-        if (lane == 0) {
-          wres[wv] = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
-          atomicMin(&found_k, k);
-          atomicMin(&stop_k, k);
+    bool direct = false;
+    for (u32 phase = 0;; ++phase) {
+      for (;;) {
+        help_split(&rs, seen, wv, lane);
+        u32 k = 0;
+        if (lane == 0) k = atomicAdd(&next_k, 1u);
+        k = __shfl(k, 0, 64);
+        if (k >= (phase ? kDirectPairs : nj) || k > *stop) break;
+        u64 A;
+        const u16 *small;
+        u32 ns, r = 0, mg = 0, c32 = 0, qm = 0;
+        if (!phase) {
+          r = r0 + 4 * k;
+          A = A0 + k;
+          mg = r == 1 ? 0xFFFFFFFFu : (u32)(0x100000000ull / r);
+          c32 = (u32)(0x100000000ull % r);
+          qm = (small_mod(p, r, mg, c32) * small_mod(A, r, mg, c32)) % r;
+          u32 ga = qm, gb = r;
+          while (gb) { const u32 t = ga % gb; ga = gb; gb = t; }
+          if (r > 1 && ga != 1) continue;
+          ns = nfac[k];
+          small = fac[k];
+        } else {
+          const u64 a = k / 8 + 1, d = k % 8 + 1;
+          const u128 nn = (u128)p + (u128)(4 * a * a * d);
+          if (nn > kDirectNMaxDev) continue;
+          A = (u64)nn;
+          ns = wave_trial(A, T, fac[wv], lane);
+          small = fac[wv];
         }
-        break;
+        const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, small, ns, T, F[wv], lane, stop, k, &rs, board);
+        if (fr == kFacStop) break;
+        if (fr == kFacHost) {
+          if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
+          break;
+        }
+        bool hit;
+        RungOut mine;
+        if (!phase) {
+          const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], lane);
+          hit = u != ~(u128)0;
+          mine = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
+        } else {
+          const u64 a = k / 8 + 1, d = k % 8 + 1;
+          u64 c = 0, kq = 0;
+          hit = wave_direct_divisor(A, 4 * a * d, a, F[wv], lane, c, kq);
+          mine = RungOut{c, kq, (u64)k, (u32)(a | (d << 8)), kRungDirect};
+        }
+        if (hit) {
+          if (lane == 0) {
+            wres[wv] = mine;
+            atomicMin(&found_k, k);
+            atomicMin(&stop_k, k);
+          }
+          break;
+        }
       }
+      // A wave done with its tasks helps whichever wave still splits a cofactor; all leave
+      // once every wave is done (each task loop ends: its rho is bounded).
+      if (lane == 0) atomicAdd(&rs.done, 1u);
+      while (__builtin_amdgcn_readfirstlane(((volatile RhoShare *)&rs)->done) < kRungWaves)
+        if (!help_split(&rs, seen, wv, lane)) __builtin_amdgcn_s_sleep(2);
+      __syncthreads();
+      if (phase || found_k != kRungSlots || host_k != kRungSlots) break;   // uniform after the barrier
+      // ---- on to stage E: every wave has read found_k and host_k before they are reset
+      direct = true;
+      __syncthreads();
+      if (tid == 0) {
+        next_k = 0;
+        found_k = host_k = stop_k = kDirectPairs;
+        rs = RhoShare{0, 0, 0, 0, 0, 0};
+      }
+      if (tid < kRungWaves) wres[tid] = RungOut{0, 0, 0, 0, kRungNone};
+      __syncthreads();
+      seen = 0;
     }
-    // A wave done with its rungs helps whichever wave still splits a cofactor; all leave
-    // once every wave is done with its rungs (each rung loop ends: its rho is bounded).
-    if (lane == 0) atomicAdd(&rs.done, 1u);
-    while (__builtin_amdgcn_readfirstlane(((volatile RhoShare *)&rs)->done) < kRungWaves)
-      if (!help_split(&rs, seen, wv, lane)) __builtin_amdgcn_s_sleep(2);
-    __syncthreads();
     if (tid == 0) {
       RungOut res{0, 0, 0, 0, kRungNone};
       if (host_k < found_k) {
         res.status = kRungHost;
-      } else if (found_k < kRungSlots) {
+      } else if (direct && found_k < kDirectPairs) {
+        for (u32 w = 0; w < kRungWaves; ++w)
+          if (wres[w].status == kRungDirect && wres[w].u_hi == found_k)
+            res = RungOut{wres[w].A, wres[w].u_lo, 0, wres[w].r, kRungDirect};
+      } else if (!direct && found_k < kRungSlots) {
         for (u32 w = 0; w < kRungWaves; ++w)
           if (wres[w].status == kRungFound && wres[w].r == r0 + 4 * found_k) res = wres[w];
       }
@@ -1737,7 +1848,8 @@ int DeviceCtx::enqueue_batch(int si) {
       hipExtLaunchKernelGGL(dev::k_rung, dim3(std::max(1, cus_)), dim3(64 * dev::kRungWaves), 0, st,
                             timed ? b.ev6 : nullptr, timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p,
                             (const u32 *)b.d_scalars, (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, m.trial,
-                            b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3, b.d_stamps + 4);
+                            b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3, b.d_stamps + 4,
+                            dev::kRungSlots);
     } else {
       hipExtLaunchKernelGGL(dev::k_mr_bases, dim3(std::max(1, cus_ * 4)), dim3(256), 0, st, timed ? b.ev6 : nullptr,
                             timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
@@ -2001,7 +2113,7 @@ int DeviceCtx::run(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve,
 
 int DeviceCtx::in_flight() const { return impl_->busy; }
 
-int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
+int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out, bool direct_only) {
   Impl &m = *impl_;
   if (device_ < 0) { err_ = "device not open"; return -1; }
   if (n == 0) return 0;
@@ -2024,7 +2136,8 @@ int DeviceCtx::rung(const u64 *vals, size_t n, RungResult *out) {
   HIP_TRY(hipMemcpyAsync(d_n, n32, 16, hipMemcpyHostToDevice, m.stream));
   HIP_TRY(hipMemsetAsync(d_b, 0, dev::kBoardSlots * sizeof(dev::RhoBoard), m.stream));
   dev::k_rung<<<std::max(1, cus_), 64 * dev::kRungWaves, 0, m.stream>>>(d_v, d_n, n32[0], d_f, nullptr, m.trial, d_o,
-                                                                         d_n + 1, d_b, d_n + 3);
+                                                                         d_n + 1, d_b, d_n + 3, nullptr,
+                                                                         direct_only ? 0u : dev::kRungSlots);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d_o, n * sizeof(dev::RungOut), hipMemcpyDeviceToHost, m.stream));
   HIP_TRY(hipStreamSynchronize(m.stream));

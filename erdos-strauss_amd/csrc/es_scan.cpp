@@ -601,6 +601,10 @@ void Scanner::finish_records(const std::vector<Interval> &wins, std::vector<Wind
       const RungResult &g = br.rung[i];
       format_rung(br.surv_p[i], g.A, g.r, ((u128)g.u_hi << 64) | g.u_lo, lines[j]);
       kinds[j] = 0;
+    } else if (!br.rung.empty() && br.rung[i].status == 4) {   // stage E on the device
+      const RungResult &g = br.rung[i];
+      format_direct(br.surv_p[i], g.r & 0xFFu, g.A, g.r >> 8, g.u_lo, lines[j]);
+      kinds[j] = 1;
     } else {
       host_j.push_back(j);
     }

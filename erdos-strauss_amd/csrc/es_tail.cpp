@@ -349,15 +349,18 @@ bool solve_direct(u64 p, std::string &line) {
         if (m % q4 != q4 - 1) continue;
         const u64 c = (m + 1) / q4, k = n / m;
         if (!c || !k || (u128)c * k <= a) continue;
-        char buf[160];
-        snprintf(buf, sizeof buf, "SOLVED %llu %llu %llu %llu %llu\n", (unsigned long long)p,
-                 (unsigned long long)a, (unsigned long long)c, (unsigned long long)d,
-                 (unsigned long long)k);
-        line = buf;
+        format_direct(p, a, c, d, k, line);
         return true;
       }
     }
   return false;
+}
+
+void format_direct(u64 p, u64 a, u64 c, u64 d, u64 k, std::string &line) {
+  char buf[160];
+  snprintf(buf, sizeof buf, "SOLVED %llu %llu %llu %llu %llu\n", (unsigned long long)p, (unsigned long long)a,
+           (unsigned long long)c, (unsigned long long)d, (unsigned long long)k);
+  line = buf;
 }
 
 void format_rung(u64 p, u64 A, unsigned r, u128 u, std::string &line) {

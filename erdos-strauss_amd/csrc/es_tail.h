@@ -33,6 +33,9 @@ int certify_prime(u64 p, std::string &line);
 // Decimal digits of v written backwards so that they end at `end`; returns the first.
 char *put_u64(char *end, u64 v);
 
+// "SOLVED p a c d k\n" (rung_scan3.cpp:639-643).
+void format_direct(u64 p, u64 a, u64 c, u64 d, u64 k, std::string &line);
+
 // "RUNG p A r u\n" (rung_scan3.cpp:594-600).
 void format_rung(u64 p, u64 A, unsigned r, u128 u, std::string &line);
 

@@ -112,6 +112,8 @@ def lib():
             "es_set_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, U64]),
             "es_certify_batch": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_void_p),
                                                 P(ctypes.c_size_t)]),
+            "es_direct_batch": (ctypes.c_int, [ctypes.c_void_p, P(U64), ctypes.c_size_t, P(ctypes.c_void_p),
+                                                P(ctypes.c_size_t)]),
             "es_choose_layout": (U64, [ctypes.c_void_p, U64]),
             "es_autotune": (ctypes.c_int, [ctypes.c_void_p, P(U64), P(U64), ctypes.c_size_t, P(ctypes.c_void_p)]),
             "es_last_timing": (ctypes.c_int, [ctypes.c_void_p, P(EsTiming)]),
@@ -304,6 +306,18 @@ class Scanner:
         text = ctypes.c_void_p()
         host = ctypes.c_size_t()
         _check(lib().es_certify_batch(self.h, _u64s(primes), len(primes), ctypes.byref(text), ctypes.byref(host)))
+        try:
+            out = ctypes.string_at(text).decode()
+        finally:
+            lib().es_free(text)
+        return out.splitlines(keepends=True), host.value
+
+    def direct_batch(self, primes):
+        """Stage E alone on the device for a list of primes: (lines, how many the host
+        finished); one SOLVED or SURVIVOR line per prime, in input order."""
+        text = ctypes.c_void_p()
+        host = ctypes.c_size_t()
+        _check(lib().es_direct_batch(self.h, _u64s(primes), len(primes), ctypes.byref(text), ctypes.byref(host)))
         try:
             out = ctypes.string_at(text).decode()
         finally:

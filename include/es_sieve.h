@@ -147,6 +147,16 @@ int es_certify(uint64_t p, char *buf, size_t cap);
  * tail (the reference keeps stage D on the host, sieve/cuda/ref_host.h:20-21). */
 int es_certify_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, size_t *on_host);
 
+/* Stage E alone on the device for n primes (k_rung with no rungs: one workgroup per
+ * prime, pairs (a, d) taken in order by its waves): "SOLVED p a c d k\n" per prime stage E
+ * certifies, "SURVIVOR p\n" per prime it cannot, in input order, into *text (free with
+ * es_free); a prime the device declines is finished by the host (*on_host counts them).
+ * Pairs whose n = p + 4a^2 d exceeds (2^32-1)^2 are skipped, as the reference's guard
+ * skips them. Replaces solve_direct3 as
+ * --check-direct drives it (rung_scan3.cpp:624-647, :798-817). In a scan, stage E runs
+ * on the device the same way for every prime stage D leaves uncertified. */
+int es_direct_batch(es_scanner *s, const uint64_t *p, size_t n, char **text, size_t *on_host);
+
 /* Stage D placement for the following scans: 1 = on the device (default), 0 = host. */
 int es_set_device_tail(es_scanner *s, int on);
 
