@@ -1431,14 +1431,20 @@ __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict
     h_f[i] = f;
     if (with_rung && f) h_r[i] = rung[i];
   }
-  // Every wave waits until its own stores to host memory have completed (a fence at system
-  // scope would also write back the whole L2, once per wave); the last workgroup to count
-  // itself done then posts the end time and, with one system-scope release, the number.
+  // The mirrors are coherent host memory (not cached in any XCD's L2). Every wave waits
+  // until its own stores have completed, then one thread per workgroup releases them at
+  // system scope before counting the workgroup done; the last workgroup to count itself
+  // done acquires the others' releases and posts the end time and, with a system-scope
+  // release, the batch's number.
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(done_blocks, 1u) == gridDim.x - 1) {
-    h_sync[2] = __builtin_amdgcn_s_memrealtime();
-    __hip_atomic_store(&h_sync[0], (u64)hdr->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (atomicAdd(done_blocks, 1u) == gridDim.x - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+      h_sync[2] = __builtin_amdgcn_s_memrealtime();
+      __hip_atomic_store(&h_sync[0], (u64)hdr->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -1506,6 +1512,11 @@ struct DevLayoutBufs {
 // a process by default; with more streams than queues, streams share a queue and their
 // batches serialise (measured: six slots ran C2 steps at 90 us against four slots' 80).
 constexpr size_t kSyncWords = 3 + dev::kStamps;
+// Every pinned buffer the device writes or reads is coherent (fine-grained): the device
+// does not keep its lines in L2. Each XCD has its own L2, and a release by the workgroup
+// that posts a batch's completion writes back only its own XCD's L2, so mirror stores
+// cached in another XCD's L2 could still be on their way when the host sees the post.
+constexpr unsigned kPinnedFlags = hipHostMallocMapped | hipHostMallocCoherent;
 
 struct Slot {
   // Each slot has its own stream, so the device may also overlap batches: one batch's
@@ -1674,15 +1685,15 @@ int DeviceCtx::open(int device) {
     HIP_TRY(hipEventCreate(&b.ev6));
     HIP_TRY(hipEventCreate(&b.ev7));
     HIP_TRY(hipMalloc(&b.d_hdr, sizeof(dev::BatchHdr)));
-    HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), hipHostMallocMapped));
+    HIP_TRY(hipHostMalloc(&b.h_hdr, sizeof(dev::BatchHdr), kPinnedFlags));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_hdr, b.h_hdr, 0));
     HIP_TRY(hipMalloc(&b.d_scalars, dev::kScalars * sizeof(u32)));
     HIP_TRY(hipMalloc(&b.d_stamps, dev::kStamps * sizeof(u64)));
-    HIP_TRY(hipHostMalloc(&b.h_sync, kSyncWords * sizeof(u64), hipHostMallocMapped));
+    HIP_TRY(hipHostMalloc(&b.h_sync, kSyncWords * sizeof(u64), kPinnedFlags));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_sync, b.h_sync, 0));
     std::memset(b.h_sync, 0, kSyncWords * sizeof(u64));
     HIP_TRY(hipMalloc(&b.d_board, dev::kBoardSlots * sizeof(dev::RhoBoard)));
-    HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), hipHostMallocMapped));
+    HIP_TRY(hipHostMalloc(&b.h_count, 4 * sizeof(u32), kPinnedFlags));
     HIP_TRY(hipHostGetDevicePointer((void **)&b.dh_count, b.h_count, 0));
   }
   // Sieve table for stage D: odd primes q below 2^16 packed as {q | (2^32 mod q) << 16,
@@ -1781,7 +1792,7 @@ static hipError_t pinned_alloc(void **h, void **d, size_t bytes) {
     if (e != hipSuccess) return e;
   }
   *h = nullptr;
-  hipError_t e = hipHostMalloc(h, bytes, hipHostMallocMapped);
+  hipError_t e = hipHostMalloc(h, bytes, kPinnedFlags);
   if (e == hipSuccess && d) e = hipHostGetDevicePointer(d, *h, 0);
   return e;
 }

@@ -84,3 +84,32 @@ def test_c4_all_windows_autotuned_graph_four_in_flight(M):
             assert sc.in_flight() == 0
         finally:
             sc.set_graph(False)
+
+
+def test_c4_results_complete_when_posted_over_many_passes():
+    """Regression: a batch's completion post must follow every result the host reads.
+    k_pack's mirrors were once pinned memory the device could cache in L2; each XCD has
+    its own L2 and the posting workgroup's release wrote back only its own, so about one
+    pipelined pass in 200 handed the host a stale survivor flag or stage-D result. With
+    coherent mirrors and a system-scope release per workgroup, 1,200 passes four in flight
+    and 300 one at a time must all equal the golden records."""
+    wins = windows()
+    gold = split_records(open(os.path.join(paths.GOLDEN, "c4", "records_w2042040.txt")).read())
+    with es.Scanner(wheel=2042040, filters=es.F204, device=0) as sc:
+        sc.autotune(wins[:64])
+        sc.set_graph(True)
+        depth, passes, bad = es.MAX_IN_FLIGHT, 1200, []
+        for _ in range(depth - 1):
+            sc.submit(wins)
+        for i in range(passes):
+            if i + depth - 1 < passes:
+                sc.submit(wins)
+            recs, _ = sc.collect()
+            if recs != gold:
+                bad.append(("in flight", i, [k for k in range(512) if recs[k] != gold[k]][:3]))
+        for i in range(300):
+            recs, _ = sc.records(wins)
+            if recs != gold:
+                bad.append(("one at a time", i, [k for k in range(512) if recs[k] != gold[k]][:3]))
+        print("passes differing:", len(bad), flush=True)
+        assert not bad, bad[:5]
