@@ -194,6 +194,11 @@ __device__ __forceinline__ void stamp_end(u64 *stamp, bool writer) {
   if (stamp && writer) atomicMax((unsigned long long *)stamp + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
+// uint4 words of the next tile's offset row each thread of k_sieve holds in registers
+// between its loads and its LDS stores (a row of up to kRowPrefetch x blockDim uint4;
+// a longer row is staged at the start of its tile instead).
+constexpr u32 kRowPrefetch = 4;
+
 template <bool kPersistent, bool kRowInLds>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
                                                 const BatchHdr *__restrict__ hdr,
@@ -206,7 +211,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
   u16 *soff = (u16 *)(lds + lds_words);
-  u32 *sc = lds + lds_words + off_words;   // [0] tile, [1] survivor base,
+  u32 *sc = lds + lds_words + off_words;   // [1] survivor base, [2] next tile,
                                            // [16..) per-wave partial sums (3 x kMaxWaves)
   u32 *s_wpos = sc + 16, *s_waft = sc + 16 + kMaxWaves, *s_wsur = sc + 16 + 2 * kMaxWaves;
   const u32 tid = threadIdx.x, bs = blockDim.x;
@@ -214,20 +219,25 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   const u32 ntiles = hdr->ntiles;
   const bool sieve = hdr->has_sieve != 0;
   stamp_start(stamp);
+  // Persistent form: workgroup g starts on tile g, and the queue hands out tiles from
+  // gridDim.x on (every workgroup claiming its first tile from one counter at once would
+  // serialize their starts on that address). Each workgroup claims its next tile one
+  // ahead (the claim's latency overlaps the current tile's init and marking) and, once
+  // the current tile's marking is done and its offset row is dead, loads the next tile's
+  // row into registers while it counts and compacts, storing it into LDS at the end of
+  // the tile.
   u32 t = blockIdx.x;
+  bool row_ready = false;   // the tile's offset row is in LDS already
   for (;;) {
-    if (kPersistent) {
-      if (tid == 0) sc[0] = atomicAdd(queue, 1u);
-      __syncthreads();
-      t = sc[0];
-    }
     if (t >= ntiles) break;
+    u32 claim = 0;
+    if (kPersistent && tid == 0) claim = gridDim.x + atomicAdd(queue, 1u);
     const TileDesc td = tiles[t];
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
     const u32 P = L->P;
     const u16 *grow = L->off + (size_t)td.lane * L->ostride;
-    if (kRowInLds) {
+    if (kRowInLds && !row_ready) {
       const uint4 *src = (const uint4 *)grow;
       uint4 *dst = (uint4 *)soff;
       for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
@@ -264,6 +274,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       }
     };
     mark(0, L->ncov);
+    if (kPersistent && !sieve && tid == 0) sc[2] = claim;
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
@@ -273,7 +284,30 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       __syncthreads();
       // ---- stage B': small primes
       mark(L->ncov, L->nf);
+      if (kPersistent && tid == 0) sc[2] = claim;
       __syncthreads();
+    }
+    // ---- marking is over and the offset row dead: load the next tile's row into
+    // registers now, store it into LDS after the compaction
+    u32 tn = ntiles, nv_next = 0;
+    uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+    if (kPersistent) {
+      tn = sc[2];
+      row_ready = false;
+      if (kRowInLds && tn < ntiles) {
+        const TileDesc tdn = tiles[tn];
+        const LayoutDev *Ln = lays + tdn.layout;
+        const u32 nv = Ln->ostride >> 3;
+        if (nv <= kRowPrefetch * bs) {
+          const uint4 *src = (const uint4 *)(Ln->off + (size_t)tdn.lane * Ln->ostride);
+          if (tid < nv) pre0 = src[tid];
+          if (tid + bs < nv) pre1 = src[tid + bs];
+          if (tid + 2 * bs < nv) pre2 = src[tid + 2 * bs];
+          if (tid + 3 * bs < nv) pre3 = src[tid + 3 * bs];
+          nv_next = nv;
+          row_ready = true;
+        }
+      }
     }
     u32 my_surv = 0;
     for (u32 c = tid; c < nc; c += bs) my_surv += __popc(chunk_bits(map32, c));
@@ -318,7 +352,15 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       }
     }
     if (!kPersistent) break;
-    __syncthreads();   // the next tile reuses the map and the scalars
+    if (row_ready) {
+      uint4 *dst = (uint4 *)soff;
+      if (tid < nv_next) dst[tid] = pre0;
+      if (tid + bs < nv_next) dst[tid + bs] = pre1;
+      if (tid + 2 * bs < nv_next) dst[tid + 2 * bs] = pre2;
+      if (tid + 3 * bs < nv_next) dst[tid + 3 * bs] = pre3;
+    }
+    __syncthreads();   // the next tile reuses the map and the scalars, and finds its row staged
+    t = tn;
   }
   // Every wave leaves the loop at the same barrier (persistent) or after the tile's last
   // barrier, so thread 0's time is the workgroup's.

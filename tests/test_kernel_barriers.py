@@ -18,30 +18,34 @@ SRC = os.path.join(paths.PKG, "csrc", "es_device.hip")
 
 # (name, a statement that starts the phase), in program order within the tile loop.
 SIEVE_PHASES = [
-    ("queue claim", "sc[0] = atomicAdd(queue, 1u);"),
-    ("claimed tile read", "t = sc[0];"),
     ("offset row staged", "dst[i] = src[i];"),
     ("mask init", "map32[c * 8 + q] ="),
     ("stage B", "mark(0, L->ncov);"),
+    ("next claim stored", "sc[2] = claim;"),
     ("count after covers", "my_after += __popc(chunk_bits(map32, c));"),
     ("stage B'", "mark(L->ncov, L->nf);"),
+    ("next claim read", "tn = sc[2];"),
     ("survivor count", "my_surv += __popc(chunk_bits(map32, c));"),
     ("wave sums stored", "s_wsur[wave] = incl;"),
     ("wave sums read", "before += s_wsur[i];"),
     ("tile base stored", "sc[1] = tot_s ? atomicAdd(surv_count, tot_s) : 0u;"),
     ("compaction", "u32 idx = sc[1] + before + incl - my_surv;"),
+    ("next row stored", "if (tid < nv_next) dst[tid] = pre0;"),
     ("next tile", "stamp_end(stamp, threadIdx.x == 0);"),   # after the loop: the next tile starts over
 ]
 # Ordered pairs (writer phase, the next phase that conflicts with it) and the LDS they share.
 SIEVE_ORDER = [
-    ("queue claim", "claimed tile read"),        # sc[0]
     ("mask init", "stage B"),                    # the byte map, the staged offset row
     ("stage B", "count after covers"),           # the byte map
     ("count after covers", "stage B'"),          # the byte map (counted before B' clears it)
     ("stage B'", "survivor count"),              # the byte map
     ("wave sums stored", "wave sums read"),      # s_wsur, s_wpos, s_waft
     ("tile base stored", "compaction"),          # sc[1]
-    ("compaction", "next tile"),                 # map, sc[0], sc[1] reused by the next tile
+    ("next claim stored", "next claim read"),    # sc[2]
+    ("stage B'", "next row stored"),             # the offset row: read by marking, then replaced
+    ("next row stored", "next tile"),            # the next tile's row, read by its marking
+    ("compaction", "next tile"),                 # map, sc[1], sc[2] reused by the next tile
+    ("next claim read", "next tile"),            # sc[2], rewritten during the next tile
 ]
 
 RUNG_PHASES = [
