@@ -157,18 +157,34 @@ __device__ __forceinline__ void mark_range(u8 *map, u32 nb, const LayoutDev *L, 
   }
 }
 
-// Alive positions of one 32-position chunk as a bit word (map bytes are 0 or 1).
+// Four map bytes (0 or 1 each) as four bits.
+__device__ __forceinline__ u32 nibble(u32 x) { return (((x & 0x01010101u) * 0x01020408u) >> 24) & 0xFu; }
+
+// Alive positions of one 32-position chunk as a bit word, from two 16-byte LDS reads.
 __device__ __forceinline__ u32 chunk_bits(const u32 *map32, u32 c) {
-  u32 w = 0;
-#pragma unroll
-  for (u32 q = 0; q < 8; ++q) w |= ((((map32[c * 8 + q] & 0x01010101u) * 0x01020408u) >> 24) & 0xFu) << (4 * q);
-  return w;
+  const uint4 a = ((const uint4 *)map32)[2 * c], b = ((const uint4 *)map32)[2 * c + 1];
+  return nibble(a.x) | nibble(a.y) << 4 | nibble(a.z) << 8 | nibble(a.w) << 12 | nibble(b.x) << 16 |
+         nibble(b.y) << 20 | nibble(b.z) << 24 | nibble(b.w) << 28;
 }
 
-__device__ __forceinline__ u32 wave_sum(u32 v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Inclusive prefix sum across the 64 lanes of a wave with DPP moves (no LDS): within
+// each row of 16 lanes by shifts of 1, 2, 3, then 4 and 8 onto the upper banks, then
+// lane 15 of rows 0 and 2 into rows 1 and 3, and lane 31 into rows 2 and 3. A lane the
+// row or bank mask leaves out receives 0.
+__device__ __forceinline__ u32 wave_scan(u32 x) {
+  u32 t = x;
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   // row_shr:1
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   // row_shr:2
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x113, 0xf, 0xf, true);   // row_shr:3
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)t, 0x114, 0xf, 0xe, false);  // row_shr:4, banks 1-3
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)t, 0x118, 0xf, 0xc, false);  // row_shr:8, banks 2-3
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)t, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+  t += (u32)__builtin_amdgcn_update_dpp(0, (int)t, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
+  return t;
 }
+
+// Sum over the wave (every lane active), uniform.
+__device__ __forceinline__ u32 wave_sum(u32 v) { return (u32)__builtin_amdgcn_readlane((int)wave_scan(v), 63); }
 
 // The tile is a byte map in LDS: byte k says whether position k0 + k is still alive.
 // A mark is a plain byte store of 0 -- idempotent, so concurrent stores need no atomics.
@@ -256,8 +272,9 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
         }
         if (c == nc - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
         my_pos += __popc(word);
-#pragma unroll
-        for (u32 q = 0; q < 8; ++q) map32[c * 8 + q] = (((word >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u;
+        auto bytes = [&](u32 q) { return (((word >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u; };
+        ((uint4 *)map32)[2 * c] = make_uint4(bytes(0), bytes(1), bytes(2), bytes(3));
+        ((uint4 *)map32)[2 * c + 1] = make_uint4(bytes(4), bytes(5), bytes(6), bytes(7));
       }
     }
     __syncthreads();
@@ -314,11 +331,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     if (!sieve) my_after = my_surv;
 
     // ---- reductions and the survivor scan (inclusive within the wave)
-    u32 incl = my_surv;
-    for (u32 o = 1; o < 64; o <<= 1) {
-      const u32 n = __shfl_up(incl, o, 64);
-      if (lane64 >= o) incl += n;
-    }
+    const u32 incl = wave_scan(my_surv);
     const u32 wpos = wave_sum(my_pos), waft = wave_sum(my_after);
     if (lane64 == 63) s_wsur[wave] = incl;
     if (lane64 == 0) { s_wpos[wave] = wpos; s_waft[wave] = waft; }

@@ -19,7 +19,7 @@ SRC = os.path.join(paths.PKG, "csrc", "es_device.hip")
 # (name, a statement that starts the phase), in program order within the tile loop.
 SIEVE_PHASES = [
     ("offset row staged", "dst[i] = src[i];"),
-    ("mask init", "map32[c * 8 + q] ="),
+    ("mask init", "((uint4 *)map32)[2 * c] = make_uint4(bytes(0)"),
     ("stage B", "mark(0, L->ncov);"),
     ("next claim stored", "sc[2] = claim;"),
     ("count after covers", "my_after += __popc(chunk_bits(map32, c));"),
