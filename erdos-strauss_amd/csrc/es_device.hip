@@ -117,43 +117,40 @@ typedef const u32 __attribute__((address_space(1))) *GlobalU32;
 // Marks ordered filters [from, to), one filter per thread. Filters are sorted by
 // ascending stride, so 64 consecutive ones ("a bin") have similar hit counts. Wave w
 // takes bins w, w + nwaves, ... (heaviest first, interleaved, so the waves end close
-// together with no shared counter), and loads the next bin's offset and constants
-// before marking the current one. kWide: the tile start k0 needs 64 bits; otherwise
-// its remainders are one 32-bit reduction.
+// together with no shared counter), kMarkGroup bins at a time: it loads the group's
+// offsets and constants first, with one wait for all of them, then marks the group's
+// bins. (Loading the next bin while marking the current one does not overlap: the wait
+// for an LDS read issued before a run of stores is a wait for all of those stores too,
+// since LDS operations complete in order, so every bin would drain the wave's stores.)
+// kWide: the tile start k0 needs 64 bits; otherwise its remainders are one 32-bit
+// reduction.
+constexpr u32 kMarkGroup = 4;
+
 template <bool kWide>
 __device__ __forceinline__ void mark_range(u8 *map, u32 nb, const LayoutDev *L, const u16 *offrow, u64 k0,
                                            u32 from, u32 to, u32 wave, u32 nwaves, u32 lane64) {
   const u32 nbins = (to - from + 63u) >> 6;
   const u32 klo = (u32)k0;
   const GlobalU32 fc = (GlobalU32)L->fc;    // per filter {s, floor(2^32/s), 2^32 mod s, 0}
-  u32 b = wave;
-  u32 j = from + (b << 6) + lane64;
-  u32 o = 0xFFFFu, fs = 1, fm = 0, fr = 0;
-  if (b < nbins && j < to) {
-    o = offrow[j];
-    fs = fc[4 * j];
-    fm = fc[4 * j + 1];
-    fr = fc[4 * j + 2];
-  }
-  while (b < nbins) {
-    const u32 bn = b + nwaves;
-    const u32 jn = from + (bn << 6) + lane64;
-    u32 on = 0xFFFFu, ns = 1, nm = 0, nr = 0;
-    if (bn < nbins && jn < to) {
-      on = offrow[jn];
-      ns = fc[4 * jn];
-      nm = fc[4 * jn + 1];
-      nr = fc[4 * jn + 2];
+  for (u32 g = wave; g < nbins; g += kMarkGroup * nwaves) {
+    u32 o[kMarkGroup], fs[kMarkGroup], fm[kMarkGroup], fr[kMarkGroup];
+#pragma unroll
+    for (u32 q = 0; q < kMarkGroup; ++q) {
+      const u32 bq = g + q * nwaves, j = from + (bq << 6) + lane64;
+      const bool ok = bq < nbins && j < to;
+      const u32 jj = ok ? j : from;        // every load in bounds, so none is conditional
+      o[q] = ok ? (u32)offrow[jj] : 0xFFFFu;
+      fs[q] = fc[4 * jj];
+      fm[q] = fc[4 * jj + 1];
+      fr[q] = kWide ? fc[4 * jj + 2] : 0u;
     }
-    if (o != 0xFFFFu) {
-      const u32 km = kWide ? mod64(k0, fs, fm, fr) : mod32(klo, fs, fm);
-      clear_run(map, o >= km ? o - km : o + fs - km, fs, nb);
+#pragma unroll
+    for (u32 q = 0; q < kMarkGroup; ++q) {
+      if (o[q] != 0xFFFFu) {
+        const u32 km = kWide ? mod64(k0, fs[q], fm[q], fr[q]) : mod32(klo, fs[q], fm[q]);
+        clear_run(map, o[q] >= km ? o[q] - km : o[q] + fs[q] - km, fs[q], nb);
+      }
     }
-    b = bn;
-    o = on;
-    fs = ns;
-    fm = nm;
-    fr = nr;
   }
 }
 
@@ -210,6 +207,15 @@ __device__ __forceinline__ void stamp_end(u64 *stamp, bool writer) {
   if (stamp && writer) atomicMax((unsigned long long *)stamp + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
+// 0, in a register the compiler cannot see through. An atomic on a uniform address has its
+// result broadcast to the wave at once (so the wave waits for it right after issuing it);
+// offsetting the address by this keeps a claim issued ahead of use asynchronous.
+__device__ __forceinline__ u32 opaque_zero() {
+  u32 z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // uint4 words of the next tile's offset row each thread of k_sieve holds in registers
 // between its loads and its LDS stores (a row of up to kRowPrefetch x blockDim uint4;
 // a longer row is staged at the start of its tile instead).
@@ -235,19 +241,21 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   const u32 ntiles = hdr->ntiles;
   const bool sieve = hdr->has_sieve != 0;
   stamp_start(stamp);
-  // Persistent form: workgroup g starts on tile g, and the queue hands out tiles from
-  // gridDim.x on (every workgroup claiming its first tile from one counter at once would
-  // serialize their starts on that address). Each workgroup claims its next tile one
-  // ahead (the claim's latency overlaps the current tile's init and marking) and, once
-  // the current tile's marking is done and its offset row is dead, loads the next tile's
-  // row into registers while it counts and compacts, storing it into LDS at the end of
-  // the tile.
-  u32 t = blockIdx.x;
+  // Persistent form: workgroup g takes tiles g and gridDim.x + g first, and the queue
+  // hands out tiles from 2 gridDim.x on (every workgroup claiming from one counter at
+  // the start would serialize on that address). A workgroup claims the tile after next
+  // once the claiming wave's marking loads are done (the wave then has no load to wait
+  // for until the end of the tile, so the claim's latency stays hidden) and, once the current tile's
+  // marking is done and its offset row is dead, loads the next tile's row into registers
+  // while it counts and compacts, storing it into LDS at the end of the tile.
+  // The claimer is lane 0 of the last wave, which has no other atomic to wait for (thread
+  // 0 places the tile's survivors).
+  const u32 claimer = (nwaves - 1) * 64u;
+  u32 t = blockIdx.x, tnext = gridDim.x + blockIdx.x;
   bool row_ready = false;   // the tile's offset row is in LDS already
   for (;;) {
     if (t >= ntiles) break;
     u32 claim = 0;
-    if (kPersistent && tid == 0) claim = gridDim.x + atomicAdd(queue, 1u);
     const TileDesc td = tiles[t];
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       }
     };
     mark(0, L->ncov);
-    if (kPersistent && !sieve && tid == 0) sc[2] = claim;
+    if (kPersistent && !sieve && tid == claimer) claim = atomicAdd(queue + opaque_zero(), 1u);
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
@@ -301,18 +309,17 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       __syncthreads();
       // ---- stage B': small primes
       mark(L->ncov, L->nf);
-      if (kPersistent && tid == 0) sc[2] = claim;
+      if (kPersistent && tid == claimer) claim = atomicAdd(queue + opaque_zero(), 1u);
       __syncthreads();
     }
     // ---- marking is over and the offset row dead: load the next tile's row into
     // registers now, store it into LDS after the compaction
-    u32 tn = ntiles, nv_next = 0;
+    u32 nv_next = 0;
     uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
     if (kPersistent) {
-      tn = sc[2];
       row_ready = false;
-      if (kRowInLds && tn < ntiles) {
-        const TileDesc tdn = tiles[tn];
+      if (kRowInLds && tnext < ntiles) {
+        const TileDesc tdn = tiles[tnext];
         const LayoutDev *Ln = lays + tdn.layout;
         const u32 nv = Ln->ostride >> 3;
         if (nv <= kRowPrefetch * bs) {
@@ -372,8 +379,10 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       if (tid + 2 * bs < nv_next) dst[tid + 2 * bs] = pre2;
       if (tid + 3 * bs < nv_next) dst[tid + 3 * bs] = pre3;
     }
+    if (tid == claimer) sc[2] = 2 * gridDim.x + claim;
     __syncthreads();   // the next tile reuses the map and the scalars, and finds its row staged
-    t = tn;
+    t = tnext;
+    tnext = sc[2];
   }
   // Every wave leaves the loop at the same barrier (persistent) or after the tile's last
   // barrier, so thread 0's time is the workgroup's.
