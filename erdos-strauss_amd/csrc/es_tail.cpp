@@ -1,6 +1,7 @@
 // es_tail.cpp -- stage D/E on the host. Montgomery multiplication modulo odd n < 2^64
 // (REDC with the explicit high-half carry, so n >= 2^63 is exact), magic-inverse trial
-// division, Brent's rho, and the per-residue minimum over divisors of q^2.
+// division, Pollard's rho with Brent's cycle finding, and the per-residue minimum over
+// divisors of q^2.
 #include "es_tail.h"
 
 #include <algorithm>
@@ -127,44 +128,77 @@ struct Fac {
   }
 };
 
+// floor(sqrt(n)) by Newton's iteration from above: x0 = 2^ceil(bits/2) exceeds sqrt(n),
+// and x -> (x + n/x) / 2 decreases strictly until it reaches floor(sqrt(n)).
 u64 isqrt64(u64 n) {
-  u64 s = (u64)std::sqrt((long double)n);
-  if (s > 0xFFFFFFFFull) s = 0xFFFFFFFFull;
-  while (s && s * s > n) --s;
-  while (s < 0xFFFFFFFFull && (s + 1) * (s + 1) <= n) ++s;
-  return s;
+  if (n < 2) return n;
+  u64 x = 1ull << ((65 - __builtin_clzll(n)) / 2);
+  for (;;) {
+    const u64 y = (x + n / x) / 2;
+    if (y >= x) return x;
+    x = y;
+  }
 }
 
-// Brent's rho in Montgomery form; returns a nontrivial factor of odd composite n.
+// One rho walk x -> x^2 + c (Montgomery form) from x0, with Brent's cycle finding (R. P.
+// Brent, "An improved Monte Carlo factorization algorithm", BIT 20 (1980) 176-184): the
+// saved point moves to the walk's position at every power-of-two length, and the walk's
+// differences from it are multiplied together kRhoBatch at a time, one gcd per batch.
+// A batch whose gcd is n is retraced one step and one gcd at a time. Returns a proper
+// factor, or 0 when the walk closed its cycle modulo n itself.
+constexpr u64 kRhoBatch = 96;
+
+// gcd(a, n) for odd n by Stein's binary method (shifts and subtractions, no division).
+u64 gcd_odd(u64 a, u64 n) {
+  if (!a) return n;
+  a >>= __builtin_ctzll(a);
+  while (a != n) {
+    if (a > n) std::swap(a, n);
+    n -= a;
+    n >>= __builtin_ctzll(n);
+  }
+  return a;
+}
+
+u64 rho_walk(const Mont &M, u64 cm, u64 x0) {
+  const u64 n = M.n;
+  auto next = [&](u64 v) { return M.add(M.mul(v, v), cm); };
+  auto dist = [](u64 u, u64 v) { return u > v ? u - v : v - u; };
+  u64 x = x0;
+  for (u64 len = 1;; len <<= 1) {
+    const u64 saved = x;
+    for (u64 walked = 0; walked < len;) {
+      const u64 run = std::min(kRhoBatch, len - walked), from = x;
+      u64 prod = M.r1;
+      for (u64 i = 0; i < run; ++i) {
+        x = next(x);
+        prod = M.mul(prod, dist(saved, x));
+      }
+      u64 g = gcd_odd(prod, n);
+      if (g == n) {
+        g = 1;
+        for (u64 y = from, i = 0; i < run && g == 1; ++i) {
+          y = next(y);
+          g = gcd_odd(dist(saved, y), n);
+        }
+      }
+      if (g == n) return 0;
+      if (g != 1) return g;
+      walked += run;
+    }
+  }
+}
+
+// A proper factor of odd composite n that is not a perfect square: walks with c = 1, 3,
+// 5, ... from pseudo-random starting points until one splits n (each walk ends, since its
+// sequence modulo n is eventually periodic).
 u64 rho_factor(u64 n) {
   const Mont M(n);
-  for (u64 c = 1;; ++c) {
-    const u64 cm = M.to(c);
-    auto f = [&](u64 v) { return M.add(M.mul(v, v), cm); };
-    u64 y = M.to(2), x = y, ys = y, acc = M.r1, g = 1;
-    const u64 m = 128;
-    for (u64 r = 1; g == 1 && r <= (1ull << 26); r <<= 1) {
-      x = y;
-      for (u64 i = 0; i < r; ++i) y = f(y);
-      for (u64 k = 0; k < r && g == 1; k += m) {
-        ys = y;
-        const u64 lim = std::min(m, r - k);
-        for (u64 i = 0; i < lim; ++i) {
-          y = f(y);
-          acc = M.mul(acc, x > y ? x - y : y - x);
-        }
-        g = gcd64(acc, n);
-      }
-    }
-    if (g == n || g == 1) {               // the batch overshot: replay it one step at a time
-      g = 1;
-      for (u64 guard = 0; g == 1 && guard < 256; ++guard) {
-        ys = f(ys);
-        const u64 diff = x > ys ? x - ys : ys - x;
-        g = gcd64(diff ? diff : n, n);
-      }
-    }
-    if (g != 1 && g != n) return g;
+  u64 state = n ^ 0x243F6A8885A308D3ull;
+  for (u64 c = 1;; c += 2) {
+    state = state * 6364136223846793005ull + 1442695040888963407ull;   // Knuth's MMIX LCG
+    const u64 g = rho_walk(M, M.to(c), M.to(state >> 1));
+    if (g) return g;
   }
 }
 
@@ -321,38 +355,42 @@ bool solve_rung(u64 p, std::string &line) {
   return false;
 }
 
-// rung_scan3.cpp:621-647: n = p + 4a^2 d for a, d <= 8; the least divisor m of n with
-// m == -1 (mod 4ad) whose c = (m+1)/(4ad), k = n/m satisfy ck > a.
+// Stage E (rung_scan3.cpp:607-647): the pairs (a, d) in 1..8 x 1..8, a major. For
+// n = p + 4a^2 d (at most the ceiling kDirectNMax), the certificate is the least divisor
+// m of n with m == -1 (mod 4ad) such that c = (m + 1) / (4ad) and k = n / m have
+// c k > a, printed as "SOLVED p a c d k". The first pair with such a divisor wins.
 bool solve_direct(u64 p, std::string &line) {
-  for (u64 a = 1; a <= 8; ++a)
-    for (u64 d = 1; d <= 8; ++d) {
-      const u128 nn = (u128)p + (u128)4 * a * a * d;
-      if (nn > kDirectNMax) continue;
-      const u64 n = (u64)nn, q4 = 4 * a * d;
-      Fac f;
-      factor(n, f);
-      if (f.overflow) {
-        fprintf(stderr, "FATAL: more than 16 distinct prime factors\n");
-        abort();
+  for (unsigned pair = 0; pair < 64; ++pair) {
+    const u64 a = pair / 8 + 1, d = pair % 8 + 1, q4 = 4 * a * d;
+    const u128 wide = (u128)p + (u128)(4 * a * a * d);
+    if (wide > kDirectNMax) continue;
+    const u64 n = (u64)wide;
+    Fac f;
+    factor(n, f);
+    if (f.overflow) {
+      fprintf(stderr, "FATAL: more than 16 distinct prime factors\n");
+      abort();
+    }
+    // every divisor, each prime power times those built from the earlier primes
+    std::vector<u64> divs(1, 1);
+    for (int i = 0; i < f.k; ++i) {
+      const size_t have = divs.size();
+      u64 pw = 1;
+      for (int e = 0; e < f.e[i]; ++e) {
+        pw *= f.p[i];
+        for (size_t j = 0; j < have; ++j) divs.push_back(divs[j] * pw);
       }
-      std::vector<u64> divs{1};
-      for (int i = 0; i < f.k; ++i) {
-        const size_t base = divs.size();
-        u64 pe = 1;
-        for (int e = 1; e <= f.e[i]; ++e) {
-          pe *= f.p[i];
-          for (size_t j = 0; j < base; ++j) divs.push_back(divs[j] * pe);
-        }
-      }
-      std::sort(divs.begin(), divs.end());
-      for (u64 m : divs) {
-        if (m % q4 != q4 - 1) continue;
-        const u64 c = (m + 1) / q4, k = n / m;
-        if (!c || !k || (u128)c * k <= a) continue;
+    }
+    std::sort(divs.begin(), divs.end());
+    for (const u64 m : divs) {
+      if ((m + 1) % q4) continue;                   // m == -1 (mod 4ad), so c >= 1
+      const u64 c = (m + 1) / q4, k = n / m;        // m | n and m <= n, so k >= 1
+      if ((u128)c * k > a) {
         format_direct(p, a, c, d, k, line);
         return true;
       }
     }
+  }
   return false;
 }
 
