@@ -7,7 +7,7 @@ log-uniform over [1e9, 1.8e19], length log-uniform over [2^16, 2^26] integers
 window's record must equal the golden record for the wheel in use
 (tests/golden/c4/records_w<M>.txt) at every step. For each wheel (120120 with its runtime
 table, 2042040 with the published GPU table) the autotuner chooses the sub-wheel layout
-per window size and the launch shape on a 64-window sample; the faster wheel is the one
+per window size and the launch shape on the batch itself; the faster wheel is the one
 reported. Inputs (tables, layouts) are resident in HBM before the timed region.
 
 Forms timed on the same batch:
@@ -187,7 +187,7 @@ def c4_wheel(es, M, F, steps, warmup):
     total = sum(b - a for a, b in wins)
     gold = split_records(open(os.path.join(GOLDEN, "c4", "records_w%d.txt" % M)).read())
     sc = es.Scanner(wheel=M, filters=F, device=0)
-    report = sc.autotune(wins[:64])
+    report = sc.autotune(wins)
     block, bits = tuned_launch(report)
     sc.set_graph(True)
     ok = True

@@ -283,6 +283,13 @@ int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bi
   return ES_OK;
 }
 
+int es_set_residency(es_scanner *s, uint32_t waves_per_cu) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  if (waves_per_cu > 64) return fail(ES_ERR_ARG, "more resident waves per CU than a CU holds");
+  s->sc.tuning().launch.waves_per_cu = waves_per_cu;
+  return ES_OK;
+}
+
 int es_set_graph(es_scanner *s, int on) {
   if (!s) return fail(ES_ERR_ARG, "null scanner");
   s->sc.tuning().launch.graph = on != 0;

@@ -67,6 +67,7 @@ struct LaunchCfg {
   bool graph = false;       // replay a captured hipGraph (persistent form only)
   u32 block = 256;          // threads per workgroup
   u32 max_bits = 32768;     // largest tile in positions (LDS bytes of the byte map)
+  u32 waves_per_cu = 16;    // resident waves per CU the persistent grid is sized for (0: all that fit)
   bool device_tail = true;  // stage D on the device (k_rung); the host finishes what it declines
 };
 

@@ -45,7 +45,6 @@ namespace es {
 namespace dev {
 
 typedef unsigned char u8;
-constexpr int kMaxWaves = 16;   // up to 1024 threads per workgroup
 
 struct LayoutDev {
   u64 Mp;
@@ -186,16 +185,20 @@ __device__ __forceinline__ u32 wave_sum(u32 v) { return (u32)__builtin_amdgcn_re
 // The tile is a byte map in LDS: byte k says whether position k0 + k is still alive.
 // A mark is a plain byte store of 0 -- idempotent, so concurrent stores need no atomics.
 // All LDS is one dynamic allocation: the map at offset 0 (so a mark's address is its
-// position), then the tile's row of filter offsets, then a few scalars. lds_words and
-// off_words count the map and the offset row in 32-bit words. The offset rows (one per
-// lane, megabytes in all) do not stay in L2, so each tile streams its row into LDS with
-// coalesced 16-byte loads that overlap the mask init, instead of every bin waiting on
-// its own global load. kRowInLds = false (with off_words = 0) is the form for a row
-// that does not fit beside the map: it is read from global memory.
+// position), then the tile's row of filter offsets, then its lane's periodic wheel mask,
+// then a few scalars. lds_words, off_words and mask_words count the map, the offset row
+// and the mask row in 32-bit words. The offset rows (one per lane, megabytes in all) do
+// not stay in L2, so each tile streams its row into LDS with coalesced 16-byte loads,
+// instead of every bin waiting on its own global load. kRowInLds = false (with
+// off_words = 0) is the form for a row that does not fit beside the map: it is read from
+// global memory.
 //
-// Each tile writes its counters {positions, alive after covers, survivors} to its own
-// slot of tile_counts: no counter is shared between workgroups (the host sums them per
-// window), so only tiles with survivors touch a shared word (the survivor count).
+// Counters and survivors are placed wave by wave, with no workgroup-wide step: each wave
+// adds its {positions, alive after covers, survivors} to the tile's slot of tile_counts
+// (zeroed by k_stage) with atomics whose results nothing waits for, and a wave with
+// survivors reserves their places with one atomic on the batch's survivor count. The
+// host sums the tile slots per window; the survivor list's order depends on scheduling
+// and every output path sorts it.
 // Kernel times on the device's constant clock: the first workgroup to start lowers
 // stamp[0], the last wave (or workgroup) to finish raises stamp[1]. The host reads the
 // pair back with the batch (k_pack copies it), for eager launches and graph replays alike.
@@ -228,55 +231,60 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
                                                 uint4 *__restrict__ tile_counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
                                                 u32 *__restrict__ surv_count, u32 *__restrict__ queue,
-                                                u32 lds_words, u32 off_words, u64 *stamp) {
+                                                u32 lds_words, u32 off_words, u32 mask_words, u64 *stamp) {
   extern __shared__ u32 lds[];
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
   u16 *soff = (u16 *)(lds + lds_words);
-  u32 *sc = lds + lds_words + off_words;   // [1] survivor base, [2] next tile,
-                                           // [16..) per-wave partial sums (3 x kMaxWaves)
-  u32 *s_wpos = sc + 16, *s_waft = sc + 16 + kMaxWaves, *s_wsur = sc + 16 + 2 * kMaxWaves;
+  u32 *smask = lds + lds_words + off_words;
+  u32 *sc = smask + mask_words;   // [2] the tile after next
   const u32 tid = threadIdx.x, bs = blockDim.x;
-  const u32 lane64 = tid & 63u, wave = tid >> 6, nwaves = (bs + 63u) >> 6;
+  const u32 lane64 = tid & 63u;
   const u32 ntiles = hdr->ntiles;
   const bool sieve = hdr->has_sieve != 0;
   stamp_start(stamp);
   // Persistent form: workgroup g takes tiles g and gridDim.x + g first, and the queue
   // hands out tiles from 2 gridDim.x on (every workgroup claiming from one counter at
-  // the start would serialize on that address). A workgroup claims the tile after next
-  // once the claiming wave's marking loads are done (the wave then has no load to wait
-  // for until the end of the tile, so the claim's latency stays hidden) and, once the current tile's
-  // marking is done and its offset row is dead, loads the next tile's row into registers
-  // while it counts and compacts, storing it into LDS at the end of the tile.
-  // The claimer is lane 0 of the last wave, which has no other atomic to wait for (thread
-  // 0 places the tile's survivors).
-  const u32 claimer = (nwaves - 1) * 64u;
+  // the start would serialize on that address). A workgroup loads the next tile's
+  // descriptor at the top of the current one, claims the tile after next once the
+  // claiming wave's marking loads are done (the wave then has no load to wait for until
+  // the end of the tile, so the claim's latency stays hidden) and, once the current tile's
+  // marking is done and its offset row is dead, loads the next tile's offset and mask
+  // rows into registers while it counts and compacts, storing them into LDS at the end of
+  // the tile. The claimer is lane 0 of the last wave.
+  const u32 claimer = (((bs + 63u) >> 6) - 1u) * 64u;
   u32 t = blockIdx.x, tnext = gridDim.x + blockIdx.x;
-  bool row_ready = false;   // the tile's offset row is in LDS already
+  TileDesc td = t < ntiles ? tiles[t] : TileDesc{};
+  bool row_ready = false;   // the tile's offset and mask rows are in LDS already
   for (;;) {
     if (t >= ntiles) break;
     u32 claim = 0;
-    const TileDesc td = tiles[t];
+    TileDesc tdn{};
+    if (kPersistent && tnext < ntiles) tdn = tiles[tnext];
     const LayoutDev *L = lays + td.layout;
     const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
     const u32 P = L->P;
     const u16 *grow = L->off + (size_t)td.lane * L->ostride;
-    if (kRowInLds && !row_ready) {
-      const uint4 *src = (const uint4 *)grow;
-      uint4 *dst = (uint4 *)soff;
-      for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
+    if (!row_ready) {
+      if (kRowInLds) {
+        const uint4 *src = (const uint4 *)grow;
+        uint4 *dst = (uint4 *)soff;
+        for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
+      }
+      const u32 *msrc = L->mask + (size_t)td.lane * L->mask_words;
+      for (u32 i = tid; i < L->mask_words; i += bs) smask[i] = msrc[i];
+      __syncthreads();
     }
 
     // ---- init: wheel mask and tail mask, expanded to one byte per position
     u32 my_pos = 0;
     {
-      const u32 *mrow = L->mask + (size_t)td.lane * L->mask_words;
       const u32 kP = P == 1 ? 0u : mod64(td.k0, P, L->Pmagic, L->Pc32);
       for (u32 c = tid; c < nc; c += bs) {
         u32 word = 0xFFFFFFFFu;
         if (P != 1) {
           const u32 b = mod32(kP + 32u * c, P, L->Pmagic);
-          word = __builtin_amdgcn_alignbit(mrow[(b >> 5) + 1], mrow[b >> 5], b & 31u);
+          word = __builtin_amdgcn_alignbit(smask[(b >> 5) + 1], smask[b >> 5], b & 31u);
         }
         if (c == nc - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
         my_pos += __popc(word);
@@ -289,6 +297,7 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     // ---- stage B: certified filters. The offset row is read from LDS where it was
     // staged (kRowInLds), else from global memory.
     const bool wide = (td.k0 >> 32) != 0;
+    const u32 wave = tid >> 6, nwaves = (bs + 63u) >> 6;
     auto mark = [&](u32 from, u32 to) {
       if (kRowInLds) {
         if (wide) mark_range<true>(map, nb, L, soff, td.k0, from, to, wave, nwaves, lane64);
@@ -312,23 +321,24 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       if (kPersistent && tid == claimer) claim = atomicAdd(queue + opaque_zero(), 1u);
       __syncthreads();
     }
-    // ---- marking is over and the offset row dead: load the next tile's row into
-    // registers now, store it into LDS after the compaction
-    u32 nv_next = 0;
+    // ---- marking is over and the offset and mask rows dead: load the next tile's rows
+    // into registers now, store them into LDS after the compaction
+    u32 nv_next = 0, nm_next = 0, mpre = 0;
     uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
     if (kPersistent) {
       row_ready = false;
-      if (kRowInLds && tnext < ntiles) {
-        const TileDesc tdn = tiles[tnext];
+      if (tnext < ntiles) {
         const LayoutDev *Ln = lays + tdn.layout;
-        const u32 nv = Ln->ostride >> 3;
-        if (nv <= kRowPrefetch * bs) {
+        const u32 nv = kRowInLds ? Ln->ostride >> 3 : 0u, nm = Ln->mask_words;
+        if (nv <= kRowPrefetch * bs && nm <= bs) {
           const uint4 *src = (const uint4 *)(Ln->off + (size_t)tdn.lane * Ln->ostride);
           if (tid < nv) pre0 = src[tid];
           if (tid + bs < nv) pre1 = src[tid + bs];
           if (tid + 2 * bs < nv) pre2 = src[tid + 2 * bs];
           if (tid + 3 * bs < nv) pre3 = src[tid + 3 * bs];
+          if (tid < nm) mpre = Ln->mask[(size_t)tdn.lane * nm + tid];
           nv_next = nv;
+          nm_next = nm;
           row_ready = true;
         }
       }
@@ -337,37 +347,34 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
     for (u32 c = tid; c < nc; c += bs) my_surv += __popc(chunk_bits(map32, c));
     if (!sieve) my_after = my_surv;
 
-    // ---- reductions and the survivor scan (inclusive within the wave)
+    // ---- counters and survivors, wave by wave (inclusive survivor scan within the wave)
     const u32 incl = wave_scan(my_surv);
-    const u32 wpos = wave_sum(my_pos), waft = wave_sum(my_after);
-    if (lane64 == 63) s_wsur[wave] = incl;
-    if (lane64 == 0) { s_wpos[wave] = wpos; s_waft[wave] = waft; }
-    __syncthreads();
-    u32 before = 0, tot_s = 0;
-    for (u32 i = 0; i < nwaves; ++i) {
-      if (i < wave) before += s_wsur[i];
-      tot_s += s_wsur[i];
+    const u32 wsur = (u32)__builtin_amdgcn_readlane((int)incl, 63);
+    const u32 wpos = wave_sum(my_pos), waft = sieve ? wave_sum(my_after) : wsur;
+    if (lane64 == 0) {
+      u32 *slot = (u32 *)(tile_counts + t);
+      atomicAdd(slot, wpos);
+      if (waft) atomicAdd(slot + 1, waft);
+      if (wsur) atomicAdd(slot + 2, wsur);
     }
-    if (tid == 0) {
-      u32 tot_p = 0, tot_a = 0;
-      for (u32 i = 0; i < nwaves; ++i) { tot_p += s_wpos[i]; tot_a += s_waft[i]; }
-      tile_counts[t] = make_uint4(tot_p, tot_a, tot_s, 0u);
-      sc[1] = tot_s ? atomicAdd(surv_count, tot_s) : 0u;
-    }
-    __syncthreads();
-    if (my_surv) {
-      u32 idx = sc[1] + before + incl - my_surv;
-      const u64 c = L->lanes[td.lane], Mp = L->Mp;
-      for (u32 ch = tid; ch < nc; ch += bs) {
-        u32 x = chunk_bits(map32, ch);
-        while (x) {
-          const u32 i = __builtin_ctz(x);
-          x &= x - 1;
-          if (idx < cap) {
-            surv_p[idx] = c + Mp * (td.k0 + 32ull * ch + i);
-            surv_w[idx] = td.win;
+    if (wsur) {
+      u32 base = 0;
+      if (lane64 == 0) base = atomicAdd(surv_count, wsur);
+      base = (u32)__builtin_amdgcn_readfirstlane((int)base);
+      if (my_surv) {
+        u32 idx = base + incl - my_surv;
+        const u64 c = L->lanes[td.lane], Mp = L->Mp;
+        for (u32 ch = tid; ch < nc; ch += bs) {
+          u32 x = chunk_bits(map32, ch);
+          while (x) {
+            const u32 i = __builtin_ctz(x);
+            x &= x - 1;
+            if (idx < cap) {
+              surv_p[idx] = c + Mp * (td.k0 + 32ull * ch + i);
+              surv_w[idx] = td.win;
+            }
+            ++idx;
           }
-          ++idx;
         }
       }
     }
@@ -378,10 +385,12 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       if (tid + bs < nv_next) dst[tid + bs] = pre1;
       if (tid + 2 * bs < nv_next) dst[tid + 2 * bs] = pre2;
       if (tid + 3 * bs < nv_next) dst[tid + 3 * bs] = pre3;
+      if (tid < nm_next) smask[tid] = mpre;
     }
     if (tid == claimer) sc[2] = 2 * gridDim.x + claim;
-    __syncthreads();   // the next tile reuses the map and the scalars, and finds its row staged
+    __syncthreads();   // the next tile reuses the map and the scalars, and finds its rows staged
     t = tnext;
+    td = tdn;
     tnext = sc[2];
   }
   // Every wave leaves the loop at the same barrier (persistent) or after the tile's last
@@ -389,17 +398,17 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   stamp_end(stamp, threadIdx.x == 0);
 }
 
-// Stages one batch: zeroes the survivor count and the queue heads, and copies the batch
-// header and the tile descriptors from pinned host memory (written by the host before
-// the launch) into device memory. A kernel rather than a memset and two copies, so a
+// Stages one batch: zeroes the survivor count, the queue heads and the tile counters,
+// and copies the batch header and the tile descriptors from pinned host memory (written
+// by the host before the launch) into device memory. A kernel rather than a memset and two copies, so a
 // captured graph holds no memset or copy node and no copy engine has to be waited for
 // between batches. The tile count comes from the header itself, so one captured graph
 // serves every batch that fits the buffers; the host keeps ntiles within the buffers'
 // capacity. Every other per-batch output is written in full by the kernels.
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
-                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, u32 *__restrict__ scalars,
-                        u64 *__restrict__ board_words, u32 nboard_words, u64 *__restrict__ h_sync,
-                        u64 *__restrict__ stamps) {
+                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, uint4 *__restrict__ tile_counts,
+                        u32 *__restrict__ scalars, u64 *__restrict__ board_words, u32 nboard_words,
+                        u64 *__restrict__ h_sync, u64 *__restrict__ stamps) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const BatchHdr h = *h_hdr;
   if (g == 0) {
@@ -412,6 +421,7 @@ __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restr
   for (u32 i = g; i < nboard_words; i += stride) board_words[i] = 0;   // stage D's board of posted splits
   const u32 n = h.ntiles * (u32)(sizeof(TileDesc) / sizeof(uint2));
   for (u32 i = g; i < n; i += stride) tiles[i] = h_tiles[i];
+  for (u32 i = g; i < h.ntiles; i += stride) tile_counts[i] = make_uint4(0, 0, 0, 0);   // k_sieve adds per wave
 }
 
 // ------------------------------------------------------------------ Miller-Rabin
@@ -1005,18 +1015,18 @@ __device__ __forceinline__ u64 isqrt_dev(u64 m) {
 
 constexpr int kFacOk = 0, kFacHost = 1, kFacStop = 2;
 
-// Complete factorization of a (< 2^62) into F, given the indices (into T) of the odd
-// primes below 2^16 that divide it. kFacHost if the device declines, kFacStop if a
-// smaller rung settled meanwhile.
-__device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &T, WaveFac &F, u32 lane,
-                           const volatile u32 *stop, u32 k, RhoShare *rs, RhoBoard *board) {
+// The primes below 2^16 of a (< 2^64) into F, given the indices (into T) of the odd ones
+// that divide it (2 by a shift); their exponents on one lane each, and the cofactor, all
+// of whose prime factors exceed 65521, by one exact division. kFacHost when F would
+// overflow.
+__device__ int wave_factor_small(u64 a, const u16 *small, u32 nsmall, const TrialTab &T, WaveFac &F, u32 lane,
+                                 u64 &cof) {
   if (lane == 0) { F.k = 0; F.bad = 0; }
   wave_sync();
   u64 n = a;
   const u32 e2 = (u32)__builtin_ctzll(n);
   n >>= e2;
   if (e2) fac_add(F, 2, e2, lane);
-  // Exponent of each sieved prime on its own lane; the cofactor by one exact division.
   if (F.k + nsmall > 16) return kFacHost;
   u64 pw = 1;
   if (lane < nsmall) {
@@ -1032,44 +1042,65 @@ __device__ int wave_factor(u64 a, const u16 *small, u32 nsmall, const TrialTab &
   wave_sync();
   if (lane == 0) F.k += nsmall;
   for (int s = 32; s; s >>= 1) pw *= __shfl_xor(pw, s, 64);
-  n *= inv64(pw);
+  cof = n * inv64(pw);
   wave_sync();
-  if (n > 1) {
-    // Every prime factor of n exceeds 65521, so a part below 65537^2 is prime.
-    u64 stack[4];
-    u32 top = 0;
-    stack[top++] = n;
-    while (top) {
-      const u64 m = stack[--top];
-      if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
-      const u64 s = isqrt_dev(m);
-      if (s * s == m) { fac_add(F, s, 2, lane); continue; }
-      // Become the owner of the block's shared split if nobody holds it.
-      u32 own = 0;
-      if (lane == 0) own = atomicCAS(&rs->busy, 0u, 1u) == 0u ? 1u : 0u;
-      own = __shfl(own, 0, 64);
-      if (own && lane == 0) {
-        rs->n = m;
-        rs->factor = 0;
-        __threadfence_block();
-        atomicAdd(&rs->gen, 1u);
-      }
-      wave_sync();
-      RhoRole ro{own ? rs : nullptr, 0u, 0u, 0u, board, nullptr, 0u, -1};
-      const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k, ro) : wave_rho(m, lane, stop, k, ro);
-      if (own && lane == 0) atomicExch(&rs->busy, 0u);
-      if (ro.slot >= 0 && lane == 0) {    // the split is over: free the board slot
-        atomicExch((unsigned long long *)&board[ro.slot].factor, 0ull);
-        atomicExch((unsigned long long *)&board[ro.slot].n, 0ull);
-      }
-      if (d == kRhoStop) return kFacStop;
-      if (d == kRhoFail || top + 2 > 4) return kFacHost;
-      stack[top++] = d;
-      stack[top++] = m / d;
+  return kFacOk;
+}
+
+// Whether cofactor n (every prime factor above 65521) needs a rho split: it is neither 1,
+// nor prime (a part below 65537^2 is), nor the square of a prime.
+__device__ __forceinline__ bool needs_split(u64 n, u32 lane) {
+  if (n <= 0xFFFFFFFFull || wave_is_prime(n, lane)) return false;
+  const u64 s = isqrt_dev(n);
+  return s * s != n;
+}
+
+// The cofactor's primes into F (every prime factor of n exceeds 65521), splitting
+// composite parts with rho. kFacHost if the device declines, kFacStop if a smaller rung
+// settled meanwhile.
+__device__ int wave_split(u64 n, WaveFac &F, u32 lane, const volatile u32 *stop, u32 k, RhoShare *rs,
+                          RhoBoard *board) {
+  u64 stack[4];
+  u32 top = 0;
+  if (n > 1) stack[top++] = n;
+  while (top) {
+    const u64 m = stack[--top];
+    if (m <= 0xFFFFFFFFull || wave_is_prime(m, lane)) { fac_add(F, m, 1, lane); continue; }
+    const u64 s = isqrt_dev(m);
+    if (s * s == m) { fac_add(F, s, 2, lane); continue; }
+    // Become the owner of the block's shared split if nobody holds it.
+    u32 own = 0;
+    if (lane == 0) own = atomicCAS(&rs->busy, 0u, 1u) == 0u ? 1u : 0u;
+    own = __shfl(own, 0, 64);
+    if (own && lane == 0) {
+      rs->n = m;
+      rs->factor = 0;
+      __threadfence_block();
+      atomicAdd(&rs->gen, 1u);
     }
+    wave_sync();
+    RhoRole ro{own ? rs : nullptr, 0u, 0u, 0u, board, nullptr, 0u, -1};
+    const u64 d = m < kF64Mod ? wave_rho_f64(m, lane, stop, k, ro) : wave_rho(m, lane, stop, k, ro);
+    if (own && lane == 0) atomicExch(&rs->busy, 0u);
+    if (ro.slot >= 0 && lane == 0) {    // the split is over: free the board slot
+      atomicExch((unsigned long long *)&board[ro.slot].factor, 0ull);
+      atomicExch((unsigned long long *)&board[ro.slot].n, 0ull);
+    }
+    if (d == kRhoStop) return kFacStop;
+    if (d == kRhoFail || top + 2 > 4) return kFacHost;
+    stack[top++] = d;
+    stack[top++] = m / d;
   }
   return F.bad == 0 ? kFacOk : kFacHost;
 }
+
+// Stage D's witness search can run before a cofactor split: every divisor of (pA)^2
+// that the known primes (p, and A's primes below 2^16) do not build alone is a multiple
+// of a prime of the unsplit cofactor, so at least 65537. A witness below that over the
+// known primes is therefore the rung's least witness, and the split is not needed. (The
+// reference factors A completely first, rung_scan3.cpp:578-603; the least witness is
+// the same.)
+constexpr u128 kWitnessBound = 65537;   // the least prime above 65521
 
 __device__ __forceinline__ u32 small_mod(u64 x, u32 r, u32 mg, u32 c32) { return mod64(x, r, mg, c32); }
 
@@ -1403,23 +1434,39 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
           ns = wave_trial(A, T, fac[wv], lane);
           small = fac[wv];
         }
-        const int fr = ns > kSmallFac ? kFacHost : wave_factor(A, small, ns, T, F[wv], lane, stop, k, &rs, board);
+        u64 cof = 1;
+        int fr = ns > kSmallFac ? kFacHost : wave_factor_small(A, small, ns, T, F[wv], lane, cof);
+        // Stage D first tries the witness over the known primes when the cofactor would
+        // need a split; the split runs only if that witness is not provably least.
+        bool known_only = !phase && fr == kFacOk && needs_split(cof, lane);
+        bool hit = false;
+        RungOut mine;
+        for (;;) {
+          if (fr == kFacOk && !known_only && cof > 1) {
+            fr = wave_split(cof, F[wv], lane, stop, k, &rs, board);
+            cof = 1;
+          }
+          if (fr != kFacOk) break;
+          if (phase) {
+            const u64 a = k / 8 + 1, d = k % 8 + 1;
+            u64 c = 0, kq = 0;
+            hit = wave_direct_divisor(A, 4 * a * d, a, F[wv], lane, c, kq);
+            mine = RungOut{c, kq, (u64)k, (u32)(a | (d << 8)), kRungDirect};
+            break;
+          }
+          const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0u, (u128)p * A, best[wv], lane);
+          if (known_only && u >= kWitnessBound) {
+            known_only = false;
+            continue;
+          }
+          hit = u != ~(u128)0;
+          mine = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
+          break;
+        }
         if (fr == kFacStop) break;
         if (fr == kFacHost) {
           if (lane == 0) { atomicMin(&host_k, k); atomicMin(&stop_k, k); }
           break;
-        }
-        bool hit;
-        RungOut mine;
-        if (!phase) {
-          const u128 u = least_witness(p, F[wv], r, mg, c32, qm ? r - qm : 0, (u128)p * A, best[wv], lane);
-          hit = u != ~(u128)0;
-          mine = RungOut{A, (u64)u, (u64)(u >> 64), r, kRungFound};
-        } else {
-          const u64 a = k / 8 + 1, d = k % 8 + 1;
-          u64 c = 0, kq = 0;
-          hit = wave_direct_divisor(A, 4 * a * d, a, F[wv], lane, c, kq);
-          mine = RungOut{c, kq, (u64)k, (u32)(a | (d << 8)), kRungDirect};
         }
         if (hit) {
           if (lane == 0) {
@@ -1652,6 +1699,7 @@ struct DeviceCtx::Impl {
   size_t occ_lds = 0;
   u32 occ_off_words = 0;
   u32 off_max = 0;                  // longest offset row over the layouts (u16 entries)
+  u32 mask_max = 0;                 // longest wheel-mask row over the layouts (u32 words)
   Slot slot[DeviceCtx::kMaxInFlight];
   int head = 0;                     // the oldest batch in flight
   int busy = 0;                     // batches in flight or reserved (0 .. kMaxInFlight)
@@ -1831,8 +1879,9 @@ int DeviceCtx::add_layout(const Layout &L) {
   d.mask_words = L.mask_words;
   d.ostride = ostride;
   m.layouts.push_back(b);
-  if (ostride > m.off_max) {
-    m.off_max = ostride;
+  if (ostride > m.off_max || L.mask_words > m.mask_max) {
+    m.off_max = std::max(m.off_max, ostride);
+    m.mask_max = std::max(m.mask_max, (u32)L.mask_words);
     for (Slot &b : m.slot) b.graph_ok = false;
   }
   // Mirror the descriptor array on the device.
@@ -1896,8 +1945,9 @@ int DeviceCtx::enqueue_batch(int si) {
   const LaunchCfg &cfg = b.cfg;
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
-                                                        (uint2 *)b.d_tiles, b.d_scalars, (u64 *)b.d_board,
-                                                        dev::kBoardSlots * 4u, b.dh_sync, b.d_stamps);
+                                                        (uint2 *)b.d_tiles, b.d_tile_counts, b.d_scalars,
+                                                        (u64 *)b.d_board, dev::kBoardSlots * 4u, b.dh_sync,
+                                                        b.d_stamps);
     HIP_TRY(hipGetLastError());
     // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
     // not by separate event packets around it.
@@ -1906,12 +1956,14 @@ int DeviceCtx::enqueue_batch(int si) {
       hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<true, true> : dev::k_sieve<true, false>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
-                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, b.d_stamps);
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, m.mask_max,
+                            b.d_stamps);
     } else {
       hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<false, true> : dev::k_sieve<false, false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
                             (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
-                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, b.d_stamps);
+                            (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, m.mask_max,
+                            b.d_stamps);
     }
     HIP_TRY(hipGetLastError());
     hipExtLaunchKernelGGL(dev::k_mr_base2, dim3(std::max(1, cus_ * 2)), dim3(256), 0, st, timed ? b.ev4 : nullptr,
@@ -2040,10 +2092,10 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     // Large generated filter sets (about 77,000 marking units at mmax 10000, a 154 KB
     // row) leave no room for a long tile beside it: then k_sieve reads the row from
     // global memory instead (off_words = 0).
-    const u32 scalar_bytes = (16 + 3 * dev::kMaxWaves) * sizeof(u32);
+    const u32 scalar_bytes = 4 * sizeof(u32), mask_bytes = m.mask_max * 4;
     b.off_words = m.off_max / 2;
-    if (lds_bytes + b.off_words * 4 + scalar_bytes > lds_limit_) b.off_words = 0;
-    b.lds = lds_bytes + b.off_words * 4 + scalar_bytes;
+    if (lds_bytes + b.off_words * 4 + mask_bytes + scalar_bytes > lds_limit_) b.off_words = 0;
+    b.lds = lds_bytes + b.off_words * 4 + mask_bytes + scalar_bytes;
     if (b.lds > lds_limit_) {
       err_ = "tile of " + std::to_string(tile_max) + " positions and " + std::to_string(m.off_max) +
              " filter offsets need " + std::to_string(b.lds) + " B of LDS, above the device's " +
@@ -2064,7 +2116,12 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
       m.occ_lds = b.lds;
       m.occ_off_words = b.off_words;
     }
-    b.grid = (u32)((size_t)cus_ * m.occupancy_blocks);
+    // Fewer resident waves than fit can mark faster (measured: 16 waves per CU beat 24
+    // and 32 for 256-, 512- and 1024-thread workgroups), so the grid is sized to
+    // cfg.waves_per_cu when that is set.
+    u32 per_cu = (u32)m.occupancy_blocks;
+    if (cfg.waves_per_cu) per_cu = std::min(per_cu, std::max(1u, cfg.waves_per_cu / ((cfg.block + 63u) / 64u)));
+    b.grid = (u32)cus_ * per_cu;
     if (enqueue_batch(si)) return -1;
   }
   release.keep = true;

@@ -260,16 +260,20 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
   if (!sample.empty()) {
     LaunchCfg best_cfg = tune_.launch;
     double best_ms = 1e300;
-    for (u32 block : {128u, 256u, 512u})
-      for (u32 bits : {8192u, 16384u, 32768u}) {
-        tune_.launch.block = block;
-        tune_.launch.max_bits = bits;
-        double ms;
-        if (time_batch(sample, ms)) { tune_ = saved; return -1; }
-        snprintf(line, sizeof line, "launch block=%u tile_bits=%u sieve_ms=%.4f\n", block, bits, ms);
-        report += line;
-        if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
-      }
+    for (u32 block : {128u, 256u, 512u, 1024u})
+      for (u32 bits : {8192u, 16384u, 32768u})
+        for (u32 waves : {16u, 24u, 0u}) {
+          if (waves && waves < block / 64) continue;
+          tune_.launch.block = block;
+          tune_.launch.max_bits = bits;
+          tune_.launch.waves_per_cu = waves;
+          double ms;
+          if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+          snprintf(line, sizeof line, "launch block=%u tile_bits=%u waves_per_cu=%u sieve_ms=%.4f\n", block, bits,
+                   waves, ms);
+          report += line;
+          if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
+        }
     tune_.launch = best_cfg;
     // The layouts above were timed at the default launch shape, and close ones can rank
     // the other way at the tuned shape (one 2^34 window at 10^15: wheel 2042040 is 14%
@@ -305,7 +309,8 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     snprintf(line, sizeof line, "choice W=2^%.0f Mp=%llu\n", e.first, (unsigned long long)e.second);
     report += line;
   }
-  snprintf(line, sizeof line, "choice block=%u tile_bits=%u\n", tune_.launch.block, tune_.launch.max_bits);
+  snprintf(line, sizeof line, "choice block=%u tile_bits=%u waves_per_cu=%u\n", tune_.launch.block,
+           tune_.launch.max_bits, tune_.launch.waves_per_cu);
   report += line;
   return 0;
 }

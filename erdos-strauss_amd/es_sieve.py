@@ -106,6 +106,7 @@ def lib():
             "es_certify": (ctypes.c_int, [U64, ctypes.c_char_p, ctypes.c_size_t]),
             "es_set_layout": (ctypes.c_int, [ctypes.c_void_p, U64]),
             "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),
+            "es_set_residency": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
             "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "es_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
             "es_set_device_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -290,6 +291,9 @@ class Scanner:
 
     def set_launch(self, persistent=True, block=256, max_bits=32768):
         _check(lib().es_set_launch(self.h, 1 if persistent else 0, block, max_bits))
+
+    def set_residency(self, waves_per_cu=16):
+        _check(lib().es_set_residency(self.h, waves_per_cu))
 
     def set_graph(self, on=True):
         _check(lib().es_set_graph(self.h, 1 if on else 0))

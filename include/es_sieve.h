@@ -178,6 +178,12 @@ int es_set_layout(es_scanner *s, uint64_t Mp);
  * size (rung_scan3.cpp:999). */
 int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bits);
 
+/* Resident waves per CU that the persistent grid is sized for (default 16; 0 = as many
+ * workgroups as fit). Fewer resident waves than fit can mark faster. No counterpart in
+ * the reference (one grid of ceil(tiles / block) workgroups, sieve/cuda/scan.cu:212-221);
+ * output does not depend on it. */
+int es_set_residency(es_scanner *s, uint32_t waves_per_cu);
+
 /* Replays the persistent form's device pipeline (counter reset, marking, Miller-Rabin)
  * from one captured hipGraph (1) or launches it kernel by kernel (0, the default).
  * No counterpart in the reference, which launches per round (sieve/cuda/scan.cu:212-221). */
@@ -188,10 +194,11 @@ int es_set_tuning(es_scanner *s, double c_pair, double c_mark, double c_bit);
 
 /* The project's autotuner: times every candidate sub-wheel layout on the device for
  * window sizes 2^16..2^34 (at the sample's median height) and keeps the fastest per
- * size, then picks the workgroup size and tile size that run the sample batch (nwin
- * windows [lo[i], hi[i]), may be 0) fastest. *report (free with es_free, optional)
- * receives the measurements. Output never depends on the choices. No counterpart in the
- * reference, whose launch shape is fixed by --seg/--block (sieve/cuda/scan.cuh:9-14). */
+ * size, then picks the workgroup size, tile size and resident waves per CU that run
+ * the sample batch (nwin windows [lo[i], hi[i]), may be 0) fastest. *report (free with
+ * es_free, optional) receives the measurements. Output never depends on the choices. No
+ * counterpart in the reference, whose launch shape is fixed by --seg/--block
+ * (sieve/cuda/scan.cuh:9-14). */
 int es_autotune(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin, char **report);
 
 /* Layout the autotuner picks for a window of w integers. */

@@ -181,20 +181,23 @@ def test_survivors_every_layout(name, lo, hi, kw):
             assert c[key] == want_c[key], (name, Mp, key)
 
 
-@pytest.mark.parametrize("persistent,block,bits", [(True, 64, 32), (False, 128, 1024), (True, 512, 4096),
-                                                     (False, 1024, 61440), (True, 256, 32768)])
-def test_launch_shapes(persistent, block, bits):
-    """Workgroup size, tile size and the persistent queue do not change a byte."""
+@pytest.mark.parametrize("persistent,block,bits,waves", [(True, 64, 32, 1), (False, 128, 1024, 16), (True, 512, 4096, 0),
+                                                           (False, 1024, 61440, 16), (True, 256, 32768, 24),
+                                                           (True, 1024, 16384, 16)])
+def test_launch_shapes(persistent, block, bits, waves):
+    """Workgroup size, tile size, resident waves per CU and the persistent queue do not
+    change a byte."""
     lo, hi = 10 ** 13, 10 ** 13 + 2 * 10 ** 9
     kw = dict(wheel=2042040, mmax=10000, sieve=1000, filters=F204)
     want, _ = oracle_record(lo, hi, kw)
     sc = scanner(**kw)
-    print("shape", persistent, block, bits, flush=True)
+    print("shape", persistent, block, bits, waves, flush=True)
     for Mp in (840, 2042040):
         sc.set_layout(Mp)
         sc.set_launch(persistent, block, bits)
+        sc.set_residency(waves)
         (rec,), _ = sc.records([(lo, hi)])
-        assert rec == want, (persistent, block, bits, Mp)
+        assert rec == want, (persistent, block, bits, waves, Mp)
 
 
 def test_graph_replays_match_eager():

@@ -18,33 +18,30 @@ SRC = os.path.join(paths.PKG, "csrc", "es_device.hip")
 
 # (name, a statement that starts the phase), in program order within the tile loop.
 SIEVE_PHASES = [
-    ("offset row staged", "dst[i] = src[i];"),
+    ("rows staged", "dst[i] = src[i];"),
+    ("mask row staged", "smask[i] = msrc[i];"),
     ("mask init", "((uint4 *)map32)[2 * c] = make_uint4(bytes(0)"),
     ("stage B", "mark(0, L->ncov);"),
     ("count after covers", "my_after += __popc(chunk_bits(map32, c));"),
     ("stage B'", "mark(L->ncov, L->nf);"),
     ("survivor count", "my_surv += __popc(chunk_bits(map32, c));"),
-    ("wave sums stored", "s_wsur[wave] = incl;"),
-    ("wave sums read", "before += s_wsur[i];"),
-    ("tile base stored", "sc[1] = tot_s ? atomicAdd(surv_count, tot_s) : 0u;"),
-    ("compaction", "u32 idx = sc[1] + before + incl - my_surv;"),
-    ("next row stored", "if (tid < nv_next) dst[tid] = pre0;"),
+    ("compaction", "u32 x = chunk_bits(map32, ch);"),
+    ("next rows stored", "if (tid < nv_next) dst[tid] = pre0;"),
     ("claim stored", "sc[2] = 2 * gridDim.x + claim;"),
     ("claim read", "tnext = sc[2];"),
     ("next tile", "stamp_end(stamp, threadIdx.x == 0);"),   # after the loop: the next tile starts over
 ]
 # Ordered pairs (writer phase, the next phase that conflicts with it) and the LDS they share.
 SIEVE_ORDER = [
-    ("mask init", "stage B"),                    # the byte map, the staged offset row
+    ("mask row staged", "mask init"),            # the mask row (and the offset row) of a tile staged at its top
+    ("mask init", "stage B"),                    # the byte map
     ("stage B", "count after covers"),           # the byte map
     ("count after covers", "stage B'"),          # the byte map (counted before B' clears it)
     ("stage B'", "survivor count"),              # the byte map
-    ("wave sums stored", "wave sums read"),      # s_wsur, s_wpos, s_waft
-    ("tile base stored", "compaction"),          # sc[1]
     ("claim stored", "claim read"),              # sc[2]: the tile after next
-    ("stage B'", "next row stored"),             # the offset row: read by marking, then replaced
-    ("next row stored", "next tile"),            # the next tile's row, read by its marking
-    ("compaction", "next tile"),                 # map, sc[1], sc[2] reused by the next tile
+    ("stage B'", "next rows stored"),            # the offset and mask rows: read by init and marking, then replaced
+    ("next rows stored", "next tile"),           # the next tile's rows, read by its init and marking
+    ("compaction", "next tile"),                 # the map and sc[2], reused by the next tile
 ]
 
 RUNG_PHASES = [
