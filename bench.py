@@ -36,7 +36,7 @@ Also reported, in the same JSON line:
                and work separated; cpu_baseline_port: the oracle over all 512 windows
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-legs]
-                       [--no-tuned] [--c5-seconds S] [--calib-copy MIB]
+                       [--no-tuned] [--c5-seconds S] [--calib-copy MIB] [--only WORKLOAD]
 """
 import argparse
 import hashlib
@@ -55,6 +55,7 @@ GOLDEN = os.path.join(REPO, "tests", "golden")
 TOP = 18446744065119614976                     # the stage-E ceiling (DIRECT_PMAX)
 NOMINAL_LDS_BPS = 256 * 128 * 2.4e9            # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9              # one wave-instruction per SIMD per clock
+es_mod = None                                  # the binding (erdos-strauss_amd/es_sieve.py), loaded by main()
 
 
 def load(name):
@@ -108,31 +109,40 @@ def tuned_launch(report):
     return block, bits
 
 
+def blob(records):
+    """Records as the library returns a batch: one byte string and each window's length."""
+    return "".join(records).encode("ascii"), [len(r) for r in records]
+
+
 def pipelined(sc, batch, steps, want=None, depth=4):
     """`steps` passes over one batch with up to `depth` in flight (pass i+depth-1 is
     submitted before pass i is collected): the wall time and whether every pass's records
-    equal `want` (when given)."""
+    equal `want` (when given), compared byte for byte, window by window."""
     ok = True
+    b = es_mod.Batch(batch)
+    wraw, wlens = blob(want) if want is not None else (None, None)
     t0 = time.perf_counter()
     for _ in range(min(depth - 1, steps)):
-        sc.submit(batch)
+        sc.submit(b)
     for i in range(steps):
         if i + depth - 1 < steps:
-            sc.submit(batch)
-        recs, _ = sc.collect()
+            sc.submit(b)
+        raw, lens, _ = sc.collect_raw()
         if want is not None:
-            ok = ok and recs == want
+            ok = ok and raw == wraw and lens == wlens
     return time.perf_counter() - t0, ok
 
 
 def one_at_a_time(sc, batch, steps, want):
     lat, ok = [], True
+    b = es_mod.Batch(batch)
+    wraw, wlens = blob(want)
     t0 = time.perf_counter()
     for _ in range(steps):
         s = time.perf_counter()
-        recs, _ = sc.records(batch)
+        raw, lens, _ = sc.records_raw(b)
         lat.append((time.perf_counter() - s) * 1e3)
-        ok = ok and recs == want
+        ok = ok and raw == wraw and lens == wlens
     return time.perf_counter() - t0, lat, ok
 
 
@@ -435,6 +445,8 @@ def main():
                     help="seconds of the serving trace to replay in real time (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a kernel of known size for scaling HBM counters)")
+    ap.add_argument("--only", choices=("c4w2042040", "c4w120120", "c1", "c2", "c3"),
+                    help="run one workload alone and print its figures (for profiling runs), not the bench line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve shards into independent intervals with no "
@@ -447,8 +459,18 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    es = load("es_sieve")
+    global es_mod
+    es = es_mod = load("es_sieve")
     records_mod = load("records")
+    if args.only:
+        legs = {"c4w2042040": lambda: c4_wheel(es, 2042040, es.F204, args.steps, args.warmup),
+                "c4w120120": lambda: c4_wheel(es, 120120, es.F120, args.steps, args.warmup),
+                "c1": lambda: plain_leg(es, records_mod, "c1", 120120, es.F120, 255255, 255255 + 10 ** 8, args.steps, 0),
+                "c2": lambda: plain_leg(es, records_mod, "c2", 2042040, es.F204, 10 ** 15, 10 ** 15 + 2 ** 34,
+                                        args.steps, 0),
+                "c3": lambda: plain_leg(es, records_mod, "c3", 2042040, es.F204, TOP - 2 ** 34, TOP, args.steps, 0)}
+        print(json.dumps({"only": args.only, "figures": legs[args.only]()}))
+        return
 
     by_wheel = {M: c4_wheel(es, M, F, args.steps, args.warmup)
                 for M, F in ((2042040, es.F204), (120120, es.F120))}
@@ -465,7 +487,7 @@ def main():
     roof["frac_of_measured"] = roof["achieved"] / roof["peak_measured_lds_and"]
     roof["scattered_byte_stores_per_s"] = scatter
     roof["frac_of_scattered_store_rate"] = roof["marks_per_s"] / scatter
-    valu = profile_value("k_rung_c4_valu_counters.txt", "valu_wave_insts_per_dispatch_w%d" % wheel)
+    valu = profile_value("k_sieve_rung_sq_counters.txt", "valu_wave_insts_per_dispatch_w%d" % wheel)
     rung_ms = h["split_ms"]["rung_ms"]
     roof["k_rung"] = {"bound": "valu", "kernel": "k_rung (stage D)", "unit": "wave-instructions/s",
                       "peak": VALU_ISSUE_PEAK, "kernel_ms": rung_ms, "valu_wave_insts_per_dispatch": valu,

@@ -453,25 +453,11 @@ __device__ __forceinline__ double mulmod_sgn(double a, double b, double nd, doub
 
 constexpr u64 kF64Mod = 1ull << 50;
 
-// n odd > 37; d odd with n - 1 = d 2^s; base a small.
+// n odd > 37; d odd with n - 1 = d 2^s; base a small; one = 2^64 mod n. The windowed
+// left-to-right test of es_montgomery.h (one Montgomery squaring per exponent bit, and a
+// multiplication by a small power of a per digit).
 __device__ bool strong_probable(u64 n, u64 ninv, u64 one, u64 a, u64 d, u32 s) {
-  u64 am = 0;                                 // a * 2^64 mod n by double-and-add
-  for (int bit = 5; bit >= 0; --bit) {
-    am = add_mod(am, am, n);
-    if ((a >> bit) & 1) am = add_mod(am, one, n);
-  }
-  const u64 minus_one = n - one;
-  u64 x = one;
-  for (u64 e = d; e; e >>= 1) {
-    if (e & 1) x = mont_mul(x, am, n, ninv);
-    am = mont_mul(am, am, n, ninv);
-  }
-  if (x == one || x == minus_one) return true;
-  for (u32 i = 1; i < s; ++i) {
-    x = mont_mul(x, x, n, ninv);
-    if (x == minus_one) return true;
-  }
-  return false;
+  return mont::strong_test(n, (u32)a, d, s, (u32)ninv, one, 1.0 / (double)n);
 }
 
 // The same test for n < 2^50 in signed FP64 residues: every operand of mulmod_sgn stays

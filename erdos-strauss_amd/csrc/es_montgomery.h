@@ -61,5 +61,54 @@ ES_HD u64 neg_inv(u64 n) {
   return 0 - x;
 }
 
+// x a mod n for x < n (n odd) and 1 <= a < 2^32, with rn = 1/(double)n: two 32 x 32 -> 64
+// multiplies form x a in 128 bits; the quotient q = floor(x a / n) < a is estimated as
+// trunc((double)x * a * rn), within 2^32 * 5 * 2^-53 < 2^-18 of x a / n (every FP64 step
+// rounds by at most 2^-53 relatively), so the estimate is q - 1, q or q + 1; two more
+// multiplies form x a - q' n in 128 bits, which lies in [-n, 2n), and one correction
+// brings it into [0, n). Multiplying a Montgomery form x R by a plain integer a gives the
+// Montgomery form of x a.
+ES_HD u64 mul_small(u64 x, u32 a, u64 n, double rn) {
+  const u64 p0 = (u64)lo32(x) * a;
+  const u64 p1 = (u64)hi32(x) * a + hi32(p0);
+  const u64 tl = (p1 << 32) | lo32(p0), th = p1 >> 32;
+  const u32 q = (u32)((double)x * (double)a * rn);
+  const u64 r0 = (u64)lo32(n) * q;
+  const u64 r1 = (u64)hi32(n) * q + hi32(r0);
+  const u64 ql = (r1 << 32) | lo32(r0), qh = r1 >> 32;
+  const u64 rl = tl - ql;
+  const u64 rh = th - qh - (tl < ql ? 1u : 0u);
+  if (rh >> 63) return rl + n;               // negative: r + n, whose low word is rl + n
+  return (rh || rl >= n) ? rl - n : rl;
+}
+
+// Strong probable-prime test of odd n > a to the small base a (2 <= a <= 1625, so that
+// a^3 < 2^32), n - 1 = d 2^s with d odd; ni = -n^{-1} mod 2^32, one = 2^64 mod n,
+// rn = 1/(double)n. a^d runs left to right in Montgomery form over digits of d: w bits
+// per digit, each digit w squarings and one mul_small by a^digit (w = 5 for a = 2, whose
+// powers up to 2^31 are shifts; w = 2 otherwise, a^3 < 2^32), about 1.1 to 1.4
+// Montgomery products per exponent bit instead of the two of a right-to-left ladder.
+ES_HD bool strong_test(u64 n, u32 a, u64 d, u32 s, u32 ni, u64 one, double rn) {
+  const u32 w = a == 2 ? 5u : 2u, mask = (1u << w) - 1u;
+  const u32 a2 = a * a, pw[4] = {1u, a, a2, a2 * a};
+  const int bits = 64 - __builtin_clzll(d);
+  u64 x = one;
+  bool started = false;   // the top digit (nonzero: it holds d's leading bit) needs no squaring
+  for (int sh = ((bits + (int)w - 1) / (int)w - 1) * (int)w; sh >= 0; sh -= (int)w) {
+    if (started)
+      for (u32 i = 0; i < w; ++i) x = mul(x, x, n, ni);
+    started = true;
+    const u32 dig = (u32)(d >> sh) & mask;
+    if (dig) x = mul_small(x, a == 2 ? 1u << dig : pw[dig], n, rn);
+  }
+  const u64 minus_one = n - one;
+  if (x == one || x == minus_one) return true;
+  for (u32 i = 1; i < s; ++i) {
+    x = mul(x, x, n, ni);
+    if (x == minus_one) return true;
+  }
+  return false;
+}
+
 }  // namespace mont
 }  // namespace es

@@ -220,12 +220,12 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     ScanTiming tm;
     if (device_pass(wins, recs, br, tm, msg)) return -1;      // warm (uploads the layout)
     std::vector<double> t;
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < 5; ++r) {
       if (device_pass(wins, recs, br, tm, msg)) return -1;
       t.push_back(tm.sieve_ms > 0 ? tm.sieve_ms : tm.device_ms);
     }
     std::sort(t.begin(), t.end());
-    sieve_ms = t[1];
+    sieve_ms = t[2];
     return 0;
   };
   tune_.table.clear();

@@ -5,6 +5,7 @@ records and exit codes) as Python calls. There is no CPU fallback: opening a sca
 without a HIP device raises EsError.
 """
 import array
+import itertools
 import ctypes
 import os
 
@@ -156,22 +157,32 @@ def certify(p):
     return kind, buf.value.decode()
 
 
-def _split_text(raw, rl, n):
+def _split_text(raw, lens):
     """Records as str, cut by their byte lengths (records are ASCII: one decode, then
     slices by the same offsets)."""
     txt = raw.decode("ascii")
-    out, pos = [], 0
-    for i in range(n):
-        k = rl[i]
-        out.append(txt[pos:pos + k])
-        pos += k
-    return out
+    offs = list(itertools.accumulate(lens, initial=0))
+    return [txt[a:b] for a, b in zip(offs, offs[1:])]
 
 
 def _u64s(vals):
     # array('Q') packs a list of Python ints far faster than the ctypes array constructor.
     buf = array.array("Q", vals)
     return (U64 * len(vals)).from_buffer(buf) if vals else (U64 * 0)()
+
+
+class Batch:
+    """Windows [(lo, hi)] packed once into the u64 arrays the library reads, for a batch
+    that is submitted again and again (records(), submit() also take plain lists)."""
+
+    def __init__(self, windows):
+        self.n = len(windows)
+        self.lo = _u64s([w[0] for w in windows])
+        self.hi = _u64s([w[1] for w in windows])
+
+
+def _packed(windows):
+    return windows if isinstance(windows, Batch) else Batch(windows)
 
 
 MAX_IN_FLIGHT = 4   # ES_MAX_IN_FLIGHT of include/es_sieve.h
@@ -215,32 +226,39 @@ class Scanner:
         return lib().es_check_interval(self.h, lo, hi) == 0
 
     def records(self, windows, emit_survivors=True):
-        """[(lo, hi)] -> list of record strings (rung_scan3 stdout per window), exit codes."""
-        n = len(windows)
-        lo = _u64s([w[0] for w in windows])
-        hi = _u64s([w[1] for w in windows])
+        """[(lo, hi)] (or a Batch) -> list of record strings (rung_scan3 stdout per window),
+        exit codes."""
+        raw, lens, codes = self.records_raw(windows, emit_survivors)
+        return _split_text(raw, lens), codes
+
+    def records_raw(self, windows, emit_survivors=True):
+        """records() as the library returns them: (all records as one bytes object, each
+        window's record length in bytes, exit codes)."""
+        b = _packed(windows)
+        n = b.n
         text = ctypes.c_void_p()
         tl = ctypes.c_size_t()
         rl = (ctypes.c_size_t * max(1, n))()
         ec = (ctypes.c_int * max(1, n))()
-        _check(lib().es_scan_records(self.h, lo, hi, n, 1 if emit_survivors else 0, ctypes.byref(text),
+        _check(lib().es_scan_records(self.h, b.lo, b.hi, n, 1 if emit_survivors else 0, ctypes.byref(text),
                                      ctypes.byref(tl), rl, ec))
         try:
             raw = ctypes.string_at(text, tl.value)
         finally:
             lib().es_free(text)
-        return _split_text(raw, rl, n), [ec[i] for i in range(n)]
+        return raw, rl[:n], ec[:n]
 
     def submit(self, windows):
-        """Pipelined records(): plans [(lo, hi)] and launches the batch without waiting.
-        At most MAX_IN_FLIGHT batches are in flight; collect() returns them in submission
-        order."""
-        n = len(windows)
-        _check(lib().es_scan_submit(self.h, _u64s([w[0] for w in windows]), _u64s([w[1] for w in windows]), n))
-        self._sizes.append(n)
+        """Pipelined records(): plans [(lo, hi)] (or a Batch) and launches the batch without
+        waiting. At most MAX_IN_FLIGHT batches are in flight; collect() returns them in
+        submission order."""
+        b = _packed(windows)
+        _check(lib().es_scan_submit(self.h, b.lo, b.hi, b.n))
+        self._sizes.append(b.n)
 
-    def collect(self, emit_survivors=True):
-        """The oldest submitted batch's (records, exit codes), exactly as records() returns them."""
+    def collect_raw(self, emit_survivors=True):
+        """The oldest submitted batch as the library returns it: (all records as one bytes
+        object, each window's record length in bytes, exit codes)."""
         if not self._sizes:
             raise EsError("no batch in flight")
         n = self._sizes[0]
@@ -258,7 +276,13 @@ class Scanner:
             raw = ctypes.string_at(text, tl.value)
         finally:
             lib().es_free(text)
-        return _split_text(raw, rl, got.value), [ec[i] for i in range(got.value)]
+        k = got.value
+        return raw, rl[:k], ec[:k]
+
+    def collect(self, emit_survivors=True):
+        """The oldest submitted batch's (records, exit codes), exactly as records() returns them."""
+        raw, lens, codes = self.collect_raw(emit_survivors)
+        return _split_text(raw, lens), codes
 
     def in_flight(self):
         return lib().es_scan_in_flight(self.h)

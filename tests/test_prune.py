@@ -8,6 +8,11 @@
   published GPU table: the same summary statistics to every printed digit, and the same
   filters up to ties between filters that kill exactly the same positions;
 - given the MI355X constants, it regenerates the committed MI355X table byte for byte.
+
+The two full-domain runs (2308 lanes x 65536 positions) take about 15 minutes each on 8
+cores, so they are opt-in (ES_LONG_TESTS=1). By default es_prune runs on a reduced domain
+(8192 positions per lane): the same table from 1 and 8 threads, every filter certified,
+and the output equal to the fixture the full-domain-validated binary made.
 """
 import hashlib
 import os
@@ -66,11 +71,32 @@ def prune(tmp_path, args):
     assert os.path.exists(PRUNE), "bin/es_prune is not built: run __graft_entry__.build()"
     out = str(tmp_path / "table.txt")
     p = subprocess.run([PRUNE] + args + ["--out", out], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                       timeout=900)
+                       timeout=3600)
     assert p.returncode == 0, p.stderr.decode()
     return out
 
 
+LONG = pytest.mark.skipif(os.environ.get("ES_LONG_TESTS") != "1", reason="15 minutes on 8 cores: set ES_LONG_TESTS=1")
+
+
+def test_pruner_reduced_domain(tmp_path, table_check):
+    want = [l for l in open(os.path.join(HERE, "golden", "prune", "ref_npos8192.sha256")) if not l.startswith("# es_prune")
+            and not l.startswith("# (a reduced") and not l.startswith("# 65536")]
+    outs = []
+    for threads in ("1", "8"):
+        d = tmp_path / threads
+        d.mkdir()
+        outs.append(open(prune(d, REF_ARGS + ["--npos", "8192", "--threads", threads])).read())
+    assert outs[0] == outs[1]
+    assert hashlib.sha256(outs[0].encode()).hexdigest() == want[0].strip()
+    assert [l for l in outs[0].splitlines(keepends=True) if l.startswith("# filters=")][0] == want[1]
+    path = str(tmp_path / "t.txt")
+    open(path, "w").write(outs[0])
+    out = subprocess.run([table_check, path, "2042040", "2000"], stdout=subprocess.PIPE, check=True)
+    assert out.stdout.decode().strip() == "ok %d" % len(filters(path))
+
+
+@LONG
 def test_pruner_reproduces_the_reference_gpu_table(tmp_path):
     got = prune(tmp_path, REF_ARGS)
     assert summary(got) == summary(GPU_TABLE)
@@ -82,6 +108,7 @@ def test_pruner_reproduces_the_reference_gpu_table(tmp_path):
     assert len(a ^ b) <= 10
 
 
+@LONG
 def test_pruner_regenerates_the_mi355x_table(tmp_path):
     got = prune(tmp_path, MI_ARGS)
     assert open(got).read() == open(MI_TABLE).read()
