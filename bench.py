@@ -12,7 +12,7 @@ reported. Inputs (tables, layouts) are resident in HBM before the timed region.
 
 Forms timed on the same batch:
   batched   all 512 windows in one persistent work-queue launch replayed from a hipGraph,
-            up to four batches in flight (one stream each): the headline `value`
+            up to eight batches in flight (one stream each): the headline `value`
   one at a time   the same, nothing overlapped: the per-batch latency p50/p99
   plain     one launch (and one host round trip) per window
 
@@ -26,7 +26,7 @@ Also reported, in the same JSON line:
   legs         the plain configurations: 1e8 integers from 255255 (wheel 120120, 96 windows
                of 2^20 in the plain form), 2^34 integers from 1e15 (4096 windows of 2^22 in
                the plain form) and the top 2^34 integers below the stage-E ceiling, each one
-               at a time and four in flight; and the MI355X-tuned filter table
+               at a time and eight in flight; and the MI355X-tuned filter table
   c5           the serving trace (Poisson 200 req/s, seed 1) replayed in real time through
                the continuous-batching server, every request's record checked against its
                committed sha256; then the whole trace at 10x, 40x and 80x its arrival rate, the
@@ -114,11 +114,12 @@ def blob(records):
     return "".join(records).encode("ascii"), [len(r) for r in records]
 
 
-def pipelined(sc, batch, steps, want=None, depth=4):
+def pipelined(sc, batch, steps, want=None, depth=None):
     """`steps` passes over one batch with up to `depth` in flight (pass i+depth-1 is
     submitted before pass i is collected): the wall time and whether every pass's records
     equal `want` (when given), compared byte for byte, window by window."""
     ok = True
+    depth = depth or es_mod.MAX_IN_FLIGHT
     b = es_mod.Batch(batch)
     wraw, wlens = blob(want) if want is not None else (None, None)
     t0 = time.perf_counter()
@@ -229,7 +230,7 @@ def c4_wheel(es, M, F, steps, warmup):
 
 def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
     """One plain configuration: the whole interval as one batched step (one at a time and
-    four in flight, graph replayed) and, in the plain form, windows of plain_window
+    eight in flight, graph replayed) and, in the plain form, windows of plain_window
     integers one launch each, merged in lane order. Records compared every step."""
     gold = open(os.path.join(GOLDEN, "records", name + ".txt")).read()
     sc = es.Scanner(wheel=M, filters=F, device=0)
@@ -272,7 +273,7 @@ def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
 def tuned_table_bench(es, steps):
     """The MI355X-tuned filter table (tables/filters/f_M2042040_mmax2000_mi355x.txt, made by
     bin/es_prune) beside the reference's GPU table on 32 consecutive 2^34 windows from
-    1e15 (four in flight, the first window's record compared with the reference scanner's
+    1e15 (eight in flight, the first window's record compared with the reference scanner's
     record for that table), and the 1e15 window alone."""
     lo, hi = 10 ** 15, 10 ** 15 + 2 ** 34
     gold = open(os.path.join(GOLDEN, "records", "c2_mi355x.txt")).read()
@@ -289,12 +290,12 @@ def tuned_table_bench(es, steps):
         sc.set_graph(True)
         r = {}
         if name == "mi355x":
-            pipelined(sc, [(lo, hi)], 4)
+            pipelined(sc, [(lo, hi)], es.MAX_IN_FLIGHT)
             wall, ok = pipelined(sc, [(lo, hi)], steps, [gold])
             r["c2"] = {"value": (hi - lo) * steps / wall, "ms_per_step": wall * 1e3 / steps, "record_identical": ok}
         recs, _ = sc.records(wins)
         first_ok = name != "mi355x" or recs[0] == gold
-        pipelined(sc, wins, 4)
+        pipelined(sc, wins, es.MAX_IN_FLIGHT)
         wall, ok = pipelined(sc, wins, 6, recs)
         wall /= 6
         r["batch32"] = {"value": (wins[-1][1] - wins[0][0]) / wall, "ms_per_batch": wall * 1e3,
@@ -516,7 +517,7 @@ def main():
                                "log-uniform over [2^16, 2^26] (%d integers)" % h["total_integers"],
                    "wheel": wheel, "table": table,
                    "form": "all 512 windows in one persistent work-queue launch replayed from a hipGraph, "
-                           "layout per window size and launch shape from the autotuner, up to four batches in "
+                           "layout per window size and launch shape from the autotuner, up to eight batches in "
                            "flight (one stream each)",
                    "layouts_Mp": h["layouts_Mp"], "block": h["block"], "tile_bits": h["tile_bits"]},
         "record_identical": h["record_identical"],

@@ -92,7 +92,7 @@ class DeviceCtx {
   int launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sieve, const LaunchCfg &cfg);
   int finish(BatchResult &out);
   int in_flight() const;
-  static constexpr int kMaxInFlight = 4;
+  static constexpr int kMaxInFlight = 8;
   // Deterministic MR of arbitrary values on the device (parity checks).
   int primality(const u64 *vals, size_t n, uint8_t *flags);
   // The same test on caller-owned device buffers, enqueued on `stream` without waiting and

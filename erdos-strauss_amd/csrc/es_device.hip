@@ -21,7 +21,7 @@
 //             Pollard rho, one persistent workgroup per CU taking survivors from a queue.
 //   k_pack    survivor records and tile counters into pinned host memory.
 //
-// Host side: DeviceCtx keeps up to four batches in flight, each in its own slot (buffers,
+// Host side: DeviceCtx keeps up to eight batches in flight, each in its own slot (buffers,
 // pinned mirrors, captured hipGraph, events) on its own stream.
 //
 // Reference behaviour: sieve/cuda/mark.cu:22-91 (k_mark), survivors.cu:20-133
@@ -1605,9 +1605,12 @@ struct DevLayoutBufs {
 // stage-D leftovers, records) while later batches run: the overlap of the reference driver
 // (sieve/cuda/scan.cu:1-6, 197-207), over whole batches. Further slots keep batches
 // queued while the host finishes one and submits the next, so the device does not idle
-// behind the host. Four slots, one stream each, match the four hardware queues HIP gives
-// a process by default; with more streams than queues, streams share a queue and their
-// batches serialise (measured: six slots ran C2 steps at 90 us against four slots' 80).
+// behind the host. Eight slots, one stream each: a 512-window batch takes about 0.54 ms
+// from its staging to its completion post when it shares the device, and the host needs
+// about 0.12 ms to collect a batch and launch the next into its slot, so four slots left
+// the device short of queued work (measured on that batch: 0.179 ms per batch four deep,
+// 0.171 six deep, 0.152 eight deep; more hardware queues than HIP's default four did not
+// change it).
 constexpr size_t kSyncWords = 3 + dev::kStamps;
 // Every pinned buffer the device writes or reads is coherent (fine-grained): the device
 // does not keep its lines in L2. Each XCD has its own L2, and a release by the workgroup

@@ -2,7 +2,7 @@
 //
 // Requests are intervals [lo, hi) whose answer is exactly the record rung_scan3 prints
 // for `rung_scan3 lo hi` with the scanner's wheel and filters (rung_scan3.cpp:1128-1162).
-// A worker thread owns the scanner and keeps up to four device batches in flight:
+// A worker thread owns the scanner and keeps up to eight device batches in flight:
 //   * admission: FIFO, bounded by request slots and by an HBM budget (a fraction of the
 //     device's memory, minus what the scanner keeps resident); each request reserves an
 //     upper bound of the device memory its slices occupy until it completes;

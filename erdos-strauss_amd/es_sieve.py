@@ -185,7 +185,7 @@ def _packed(windows):
     return windows if isinstance(windows, Batch) else Batch(windows)
 
 
-MAX_IN_FLIGHT = 4   # ES_MAX_IN_FLIGHT of include/es_sieve.h
+MAX_IN_FLIGHT = 8   # ES_MAX_IN_FLIGHT of include/es_sieve.h
 
 
 class Scanner:

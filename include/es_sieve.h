@@ -97,7 +97,7 @@ int es_scan_records(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_
  * sieve_cuda's driver, which keeps round s on the host tail while round s+1 runs on the
  * device (sieve/cuda/scan.cu:1-6, 197-207). es_scan_records, es_scan_counts,
  * es_survivors and es_autotune refuse (ES_ERR_ARG) while a batch is in flight. */
-#define ES_MAX_IN_FLIGHT 4
+#define ES_MAX_IN_FLIGHT 8
 int es_scan_submit(es_scanner *s, const uint64_t *lo, const uint64_t *hi, size_t nwin);
 int es_scan_collect(es_scanner *s, int emit_survivors, char **text, size_t *text_len,
                     size_t *rec_len, int *exit_code, size_t *nwin);

@@ -4,7 +4,7 @@
 [2^16, 2^26]) for each wheel: the autotuner chooses the sub-wheel layout per window size
 and the launch shape (workgroup and tile size) on a sample of the batch; all 512 windows
 then go through one persistent work-queue launch replayed from a captured hipGraph, and
-through submit/collect with four batches in flight, each slot's graph replayed several
+through submit/collect with eight batches in flight, each slot's graph replayed several
 times. Every window's record must equal the golden record committed for that wheel
 (tests/golden/c4/records_w<M>.txt: the oracle's records, cross-checked against the
 reference binary on every window below 1e15), byte for byte, with its exit code.
@@ -43,7 +43,7 @@ def expected_code(rec):
 
 
 @pytest.mark.parametrize("M", [2042040, 120120])
-def test_c4_all_windows_autotuned_graph_four_in_flight(M):
+def test_c4_all_windows_autotuned_graph_in_flight(M):
     wins = windows()
     assert len(wins) == 512
     gold = split_records(open(os.path.join(paths.GOLDEN, "c4", "records_w%d.txt" % M)).read())
@@ -69,8 +69,8 @@ def test_c4_all_windows_autotuned_graph_four_in_flight(M):
             t = sc.timing()
             print("wheel", M, "one batch:", t, flush=True)
             assert t["tiles"] > 0 and t["replays"] == 0
-            # Four batches in flight, each slot's graph replayed: eight passes.
-            depth, passes = es.MAX_IN_FLIGHT, 8
+            # Eight batches in flight, each slot's graph replayed: sixteen passes.
+            depth, passes = es.MAX_IN_FLIGHT, 2 * es.MAX_IN_FLIGHT
             for _ in range(depth - 1):
                 sc.submit(wins)
             for i in range(passes):
@@ -91,7 +91,7 @@ def test_c4_results_complete_when_posted_over_many_passes():
     k_pack's mirrors were once pinned memory the device could cache in L2; each XCD has
     its own L2 and the posting workgroup's release wrote back only its own, so about one
     pipelined pass in 200 handed the host a stale survivor flag or stage-D result. With
-    coherent mirrors and a system-scope release per workgroup, 1,200 passes four in flight
+    coherent mirrors and a system-scope release per workgroup, 1,200 passes eight in flight
     and 300 one at a time must all equal the golden records."""
     wins = windows()
     gold = split_records(open(os.path.join(paths.GOLDEN, "c4", "records_w2042040.txt")).read())
