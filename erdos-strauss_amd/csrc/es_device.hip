@@ -408,12 +408,15 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
                         BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, uint4 *__restrict__ tile_counts,
                         u32 *__restrict__ scalars, u64 *__restrict__ board_words, u32 nboard_words,
-                        u64 *__restrict__ h_sync, u64 *__restrict__ stamps) {
+                        u64 *__restrict__ h_sync, u64 *__restrict__ stamps, u32 *__restrict__ active) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const BatchHdr h = *h_hdr;
   if (g == 0) {
     *hdr = h;
     h_sync[1] = __builtin_amdgcn_s_memrealtime();   // the batch's start on the device clock
+    // Batches between their k_stage and the end of their k_pack: stage D's board helpers
+    // read it to make room for the other batches' kernels.
+    atomicAdd(active, 1u);
   }
   if (g < kStamps) stamps[g] = (g & 1) ? 0ull : ~0ull;   // kernel (start, end) pairs
   if (g < kScalars) scalars[g] = 0;   // survivor count, tile queue, stage-D queue, survivors finished,
@@ -725,6 +728,11 @@ struct RhoBoard {
 };
 constexpr u32 kBoardSlots = 64;           // one per lane of a wave
 constexpr u32 kBoardHelpWaves = 4;        // helper waves per workgroup: one per SIMD
+// Workgroups whose helpers stay while another batch is on the device (the others leave so
+// that its kernels find the CUs free). Every workgroup helps while the batch runs alone.
+// Measured on 512 windows near 2^64 / 1e18 with eight batches in flight: helpers in every
+// workgroup 0.150 ms per batch, in 96 0.138, in 64 0.125-0.128, in 32 or 48 0.124.
+constexpr u32 kBoardHelpGroups = 32;
 constexpr u32 kPostWindowMont = 32;       // Brent window at which a Montgomery split is posted
 constexpr u32 kPostWindowF64 = 128;       // ... and an FP64 one (its steps are cheaper)
 
@@ -753,7 +761,15 @@ struct RhoRole {
   const u32 *done;          // board helpers stop once *done >= ndone (every survivor finished)
   u32 ndone;
   int slot;
+  const u32 *active;        // non-null: the helper also stops once *active > 1
 };
+
+// Whether a board helper leaves: every survivor of its batch is finished, or (a helper
+// given `active`, the count of batches on the device) another batch is on the device.
+__device__ __forceinline__ bool helper_leaves(const u32 *done, u32 ndone, const u32 *active) {
+  return __builtin_amdgcn_readfirstlane(load_dev32(done)) >= ndone ||
+         (active && __builtin_amdgcn_readfirstlane(load_dev32(active)) > 1u);
+}
 
 // Claims a free board slot for n, or -1 if none is free: each lane reads one slot, and
 // lane 0 tries up to four of the free ones, starting from a slot that depends on n so that
@@ -779,8 +795,7 @@ __device__ __forceinline__ int board_post(RhoBoard *board, u64 n, u32 lane) {
 // does not divide n (stale, from an earlier value of the slot) is cleared.
 __device__ __forceinline__ u64 rho_coop(const RhoRole &ro, u64 n) {
   if (ro.helper == 2)
-    return (uniform64(load_dev(&ro.board[ro.slot].n)) != n ||
-            __builtin_amdgcn_readfirstlane(load_dev32(ro.done)) >= ro.ndone)
+    return (uniform64(load_dev(&ro.board[ro.slot].n)) != n || helper_leaves(ro.done, ro.ndone, ro.active))
                ? kRhoStop
                : 0;
   if (ro.board && ro.slot >= 0) {
@@ -1173,10 +1188,11 @@ __device__ bool help_split(RhoShare *rs, u32 &seen, u32 wv, u32 lane) {
 // workgroups posted until every survivor of the batch is finished. `me` numbers the
 // helper waves of the grid, so helpers start their slot scan at different slots and no
 // two of them (or any owner) share walk constants; each visit takes fresh constants.
-// Helpers leave once *done reaches ndone, which every owner's bounded work guarantees.
-__device__ void board_help(RhoBoard *board, const u32 *done, u32 ndone, u32 lane, u32 me) {
+// Helpers leave once *done reaches ndone, which every owner's bounded work guarantees, or,
+// given `active`, once another batch is on the device.
+__device__ void board_help(RhoBoard *board, const u32 *done, u32 ndone, u32 lane, u32 me, const u32 *active) {
   for (u32 visit = 0;; ++visit) {
-    if (__builtin_amdgcn_readfirstlane(load_dev32(done)) >= ndone) return;
+    if (helper_leaves(done, ndone, active)) return;
     // Each lane reads one slot; the wave joins the open split with the largest Brent
     // window (the one that has run longest), ties going to the first slot at or after `me`.
     const u64 mine = load_dev(&board[lane].n);
@@ -1195,7 +1211,7 @@ __device__ void board_help(RhoBoard *board, const u32 *done, u32 ndone, u32 lane
     // constants stay below 2^27 < n / 8 (posted values exceed 2^32), which keeps the FP64
     // walk inside the bounds its product is exact for.
     RhoRole ro{nullptr, 2u, 0u, 4096u + ((visit & 255u) * 2048u + (me & 2047u)) * (64u * kRhoWalks), board, done,
-               ndone, (int)slot};
+               ndone, (int)slot, active};
     (void)(n < kF64Mod ? wave_rho_f64(n, lane, nullptr, 0, ro) : wave_rho(n, lane, nullptr, 0, ro));
   }
 }
@@ -1262,7 +1278,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
                                                           TrialTab T, RungOut *__restrict__ out, u32 *queue,
                                                           RhoBoard *board, u32 *done, u64 *stamp = nullptr,
-                                                          u32 rung_cap = kRungSlots) {
+                                                          u32 rung_cap = kRungSlots, const u32 *active = nullptr) {
   __shared__ u128 best[kRungWaves][256];
   __shared__ WaveFac F[kRungWaves];
   __shared__ u16 fac[kRungSlots][kSmallFac];
@@ -1296,7 +1312,9 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       for (u32 i = tid; i < n_; i += blockDim.x) big |= vals[i] >= (1ull << 52) && flags[i] != kComposite;
       if (big) help_batch = 1;             // any writer writes the same value
       __syncthreads();
-      if (help_batch && wv < kBoardHelpWaves) board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv);
+      if (help_batch && wv < kBoardHelpWaves)
+        board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv,
+                   blockIdx.x >= kBoardHelpGroups ? active : nullptr);
       break;
     }
     const u8 f0 = flags[idx];
@@ -1509,13 +1527,14 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
 // The last of its workgroups to finish posts the batch's end time and number into pinned
 // host memory (h_sync[2], h_sync[0]) once every mirror store has completed, so the host
 // sees the batch complete, with every mirror written, by reading host memory alone.
-// h_sync[3 ..) receives the kernels' device-clock stamps.
+// h_sync[3 ..) receives the kernels' device-clock stamps. The last workgroup also counts
+// the batch out of `active`.
 __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict__ tile_counts,
                        const u32 *__restrict__ surv_count, u32 cap, const u64 *__restrict__ surv_p,
                        const u32 *__restrict__ surv_w, const u8 *__restrict__ flags,
                        const RungOut *__restrict__ rung, u32 with_rung, u32 *h_count, uint4 *h_tc,
                        u64 *h_p, u32 *h_w, u8 *h_f, RungOut *h_r, u32 *__restrict__ done_blocks,
-                       u64 *h_sync, const u64 *__restrict__ stamps) {
+                       u64 *h_sync, const u64 *__restrict__ stamps, u32 *__restrict__ active) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const u32 nt = hdr->ntiles, n = min(*surv_count, cap);
   if (g == 0) h_count[0] = *surv_count;
@@ -1539,6 +1558,7 @@ __global__ void k_pack(const BatchHdr *__restrict__ hdr, const uint4 *__restrict
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     if (atomicAdd(done_blocks, 1u) == gridDim.x - 1) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+      atomicSub(active, 1u);   // the batch leaves the device (k_stage counted it in)
       h_sync[2] = __builtin_amdgcn_s_memrealtime();
       __hip_atomic_store(&h_sync[0], (u64)hdr->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -1683,6 +1703,7 @@ struct DeviceCtx::Impl {
   dev::LayoutDev *d_lays = nullptr;
   size_t lays_cap = 0;
   dev::TrialTab trial{};            // odd primes below 2^16 with their inverses
+  u32 *d_active = nullptr;          // batches between their k_stage and the end of their k_pack
   int occupancy_blocks = 0;
   u32 occ_block = 0;
   size_t occ_lds = 0;
@@ -1716,7 +1737,8 @@ DeviceCtx::~DeviceCtx() {
       if (b.stream) (void)hipStreamSynchronize(b.stream);
     for (DevLayoutBufs &b : m.layouts)
       for (void *p : {b.lanes, b.mask, b.step, b.magic, b.c32, b.fc, b.off}) if (p) (void)hipFree(p);
-    for (void *p : {(void *)m.d_lays, (void *)m.trial.qm, (void *)m.trial.inv, (void *)m.trial.lim})
+    for (void *p : {(void *)m.d_lays, (void *)m.trial.qm, (void *)m.trial.inv, (void *)m.trial.lim,
+                    (void *)m.d_active})
       if (p) (void)hipFree(p);
     for (Slot &b : m.slot) {
       for (void *p : {(void *)b.d_tiles, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
@@ -1776,6 +1798,8 @@ int DeviceCtx::open(int device) {
   HIP_TRY(hipStreamCreateWithFlags(&m.capture, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&m.pev0));
   HIP_TRY(hipEventCreate(&m.pev1));
+  HIP_TRY(hipMalloc(&m.d_active, sizeof(u32)));
+  HIP_TRY(hipMemset(m.d_active, 0, sizeof(u32)));
   for (Slot &b : m.slot) {
     HIP_TRY(hipEventCreate(&b.ev0));
     HIP_TRY(hipEventCreate(&b.ev1));
@@ -1936,7 +1960,7 @@ int DeviceCtx::enqueue_batch(int si) {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
                                                         (uint2 *)b.d_tiles, b.d_tile_counts, b.d_scalars,
                                                         (u64 *)b.d_board, dev::kBoardSlots * 4u, b.dh_sync,
-                                                        b.d_stamps);
+                                                        b.d_stamps, m.d_active);
     HIP_TRY(hipGetLastError());
     // Timed: ev1/ev2 are taken by the dispatch of k_sieve itself (its start and end),
     // not by separate event packets around it.
@@ -1965,7 +1989,7 @@ int DeviceCtx::enqueue_batch(int si) {
                             timed ? b.ev6 : nullptr, timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p,
                             (const u32 *)b.d_scalars, (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, m.trial,
                             b.d_rung, b.d_scalars + 2, b.d_board, b.d_scalars + 3, b.d_stamps + 4,
-                            dev::kRungSlots);
+                            dev::kRungSlots, (const u32 *)m.d_active);
     } else {
       hipExtLaunchKernelGGL(dev::k_mr_bases, dim3(std::max(1, cus_ * 4)), dim3(256), 0, st, timed ? b.ev6 : nullptr,
                             timed ? b.ev7 : nullptr, 0, (const u64 *)b.d_surv_p, (const u32 *)b.d_scalars,
@@ -1976,7 +2000,7 @@ int DeviceCtx::enqueue_batch(int si) {
                                                    b.d_surv_p, b.d_surv_w, b.d_flags, b.d_rung,
                                                    cfg.device_tail ? 1u : 0u, b.dh_count, b.dh_tile_counts,
                                                    b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung, b.d_scalars + 4,
-                                                   b.dh_sync, (const u64 *)b.d_stamps);
+                                                   b.dh_sync, (const u64 *)b.d_stamps, m.d_active);
     HIP_TRY(hipGetLastError());
     return 0;
   };
