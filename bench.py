@@ -54,7 +54,9 @@ PKG = os.path.join(REPO, "erdos-strauss_amd")
 GOLDEN = os.path.join(REPO, "tests", "golden")
 TOP = 18446744065119614976                     # the stage-E ceiling (DIRECT_PMAX)
 NOMINAL_LDS_BPS = 256 * 128 * 2.4e9            # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
-VALU_ISSUE_PEAK = 256 * 4 * 2.4e9              # one wave-instruction per SIMD per clock
+# Wave64 VALU instructions per second: the published FP32 vector rate (157.3 TFLOP/s) over
+# 64 lanes x 2 flops of an FMA, i.e. two wave-instructions per CU per clock at 2.4 GHz.
+VALU_ISSUE_PEAK = 157.3e12 / 128
 es_mod = None                                  # the binding (erdos-strauss_amd/es_sieve.py), loaded by main()
 
 
@@ -491,7 +493,8 @@ def main():
     valu = profile_value("k_sieve_rung_sq_counters.txt", "valu_wave_insts_per_dispatch_w%d" % wheel)
     rung_ms = h["split_ms"]["rung_ms"]
     roof["k_rung"] = {"bound": "valu", "kernel": "k_rung (stage D)", "unit": "wave-instructions/s",
-                      "peak": VALU_ISSUE_PEAK, "kernel_ms": rung_ms, "valu_wave_insts_per_dispatch": valu,
+                      "peak": VALU_ISSUE_PEAK,
+                      "peak_from": "FP32 vector 157.3 TFLOP/s / (64 lanes x 2 flops)", "kernel_ms": rung_ms, "valu_wave_insts_per_dispatch": valu,
                       "achieved": valu / (rung_ms * 1e-3) if valu else None,
                       "frac": valu / (rung_ms * 1e-3) / VALU_ISSUE_PEAK if valu else None}
     surv, tiles = h["split_ms"]["survivors"], h["split_ms"]["tiles"]
