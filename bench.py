@@ -104,11 +104,13 @@ def profile_value(name, key):
 
 
 def tuned_launch(report):
-    block, bits = 256, 32768
+    """The autotuner's launch choice: {"block", "tile_bits", "waves_per_cu", "group"}."""
+    out = {}
     for l in report.splitlines():
         if l.startswith("choice block="):
-            block, bits = [int(x.split("=")[1]) for x in l.split()[1:3]]
-    return block, bits
+            out = {k: int(v) for k, v in (x.split("=") for x in l.split()[1:])}
+    out["tile_bits"] = out.pop("tile_bits", 32768)
+    return out
 
 
 def blob(records):
@@ -201,7 +203,7 @@ def c4_wheel(es, M, F, steps, warmup):
     gold = split_records(open(os.path.join(GOLDEN, "c4", "records_w%d.txt" % M)).read())
     sc = es.Scanner(wheel=M, filters=F, device=0)
     report = sc.autotune(wins)
-    block, bits = tuned_launch(report)
+    launch = tuned_launch(report)
     sc.set_graph(True)
     ok = True
     for _ in range(warmup):
@@ -214,7 +216,7 @@ def c4_wheel(es, M, F, steps, warmup):
     roof = marking_roofline(sc, wins, split["sieve_ms"])
     roof["kernel_ms_hip_events_eager"] = split["eager"]["sieve_event_ms"]
     sc.set_graph(False)
-    sc.set_launch(False, block, bits)
+    sc.set_launch(False, launch["block"], launch["tile_bits"])
     t = time.perf_counter()
     precs = [sc.records([w])[0][0] for w in wins]
     plain_s = time.perf_counter() - t
@@ -227,7 +229,7 @@ def c4_wheel(es, M, F, steps, warmup):
             "plain": {"value": total / plain_s, "windows": len(wins), "s_per_pass": plain_s,
                       "record_identical": ok_plain},
             "record_identical": ok and ok1 and ok2 and ok3, "split_ms": split, "roofline": roof,
-            "layouts_Mp": layouts, "block": block, "tile_bits": bits, "total_integers": total}
+            "layouts_Mp": layouts, "launch": launch, "total_integers": total}
 
 
 def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
@@ -237,7 +239,7 @@ def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
     gold = open(os.path.join(GOLDEN, "records", name + ".txt")).read()
     sc = es.Scanner(wheel=M, filters=F, device=0)
     report = sc.autotune([(lo, hi)])
-    block, bits = tuned_launch(report)
+    launch = tuned_launch(report)
     sc.set_graph(True)
     batch = [(lo, hi)]
     want = [gold]
@@ -254,11 +256,11 @@ def plain_leg(es, records_mod, name, M, F, lo, hi, steps, plain_window):
            "value": (hi - lo) * steps / wall, "ms_per_step": wall * 1e3 / steps,
            "one_at_a_time": {"value": (hi - lo) * steps / seq_wall, "ms_per_step": seq_wall * 1e3 / steps},
            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
-           "split_ms": split, "roofline": roof, "block": block, "tile_bits": bits,
+           "split_ms": split, "roofline": roof, "launch": launch,
            "layout_Mp": sc.choose_layout(hi - lo), "record_identical": ok1 and ok2 and ok3}
     if plain_window:
         sc.set_graph(False)
-        sc.set_launch(False, block, bits)
+        sc.set_launch(False, launch["block"], launch["tile_bits"])
         wins = [(a, min(hi, a + plain_window)) for a in range(lo, hi, plain_window)]
         sc.records([wins[0]])
         t = time.perf_counter()
@@ -522,7 +524,7 @@ def main():
                    "form": "all 512 windows in one persistent work-queue launch replayed from a hipGraph, "
                            "layout per window size and launch shape from the autotuner, up to eight batches in "
                            "flight (one stream each)",
-                   "layouts_Mp": h["layouts_Mp"], "block": h["block"], "tile_bits": h["tile_bits"]},
+                   "layouts_Mp": h["layouts_Mp"], "launch": h["launch"]},
         "record_identical": h["record_identical"],
         "latency_ms": h["latency_ms"],
         "one_at_a_time": h["one_at_a_time"],

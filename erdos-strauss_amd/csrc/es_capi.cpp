@@ -290,6 +290,13 @@ int es_set_residency(es_scanner *s, uint32_t waves_per_cu) {
   return ES_OK;
 }
 
+int es_set_grouping(es_scanner *s, uint32_t max_tiles) {
+  if (!s) return fail(ES_ERR_ARG, "null scanner");
+  if (max_tiles == 0 || max_tiles > 16) return fail(ES_ERR_ARG, "a work item groups 1 to 16 tiles");
+  s->sc.tuning().launch.group = max_tiles;
+  return ES_OK;
+}
+
 int es_set_graph(es_scanner *s, int on) {
   if (!s) return fail(ES_ERR_ARG, "null scanner");
   s->sc.tuning().launch.graph = on != 0;

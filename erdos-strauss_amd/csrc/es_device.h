@@ -69,6 +69,7 @@ struct LaunchCfg {
   u32 max_bits = 32768;     // largest tile in positions (LDS bytes of the byte map)
   u32 waves_per_cu = 16;    // resident waves per CU the persistent grid is sized for (0: all that fit)
   bool device_tail = true;  // stage D on the device (k_rung); the host finishes what it declines
+  u32 group = 4;            // most tiles one k_sieve work item groups (1: every tile alone)
 };
 
 class DeviceCtx {

@@ -65,6 +65,25 @@ constexpr u32 kStamps = 6;   // (start, end) of k_sieve, k_mr_base2, k_rung / k_
 
 struct BatchHdr {
   u32 ntiles, nwin, has_sieve, seq;   // seq: the batch's number, posted back when it completes
+  u32 nitems, pad[3];                 // nitems: k_sieve's work items (single tiles and groups)
+};
+
+// One work item of k_sieve: `count` consecutive tiles from `first` (head = tiles[first]).
+// A tile long enough is an item alone; shorter ones are grouped (up to the launch's group
+// size, at most kMaxGroup, and one tile's worth of positions), so they share one pass of
+// the per-item work (init, counts, barriers) instead of each paying for it.
+struct ItemDesc {
+  TileDesc head;
+  u32 first, count;
+};
+static_assert(sizeof(ItemDesc) == 32, "ItemDesc is shared with the host");
+constexpr u32 kMaxGroup = 16;
+
+// A sub-tile of a group as k_sieve keeps it in LDS: its descriptor and its first 32-position
+// chunk in the group's map (the sub-tiles follow one another, each from a chunk boundary).
+struct GroupSub {
+  u64 k0;
+  u32 nb, lane, win, layout, c0, pad;
 };
 
 // x mod s for s < 2^16 with mg = floor(2^32 / s) (2^32 - 1 for s = 1): the quotient
@@ -224,110 +243,214 @@ __device__ __forceinline__ u32 opaque_zero() {
 // a longer row is staged at the start of its tile instead).
 constexpr u32 kRowPrefetch = 4;
 
+// Lanes 0..ns-1 of one wave store a group's table from their sub-tile descriptors
+// (every lane of the wave calls it: the chunk offsets are a scan across the wave).
+__device__ __forceinline__ void stage_sub(GroupSub *tab, const TileDesc &d, u32 ns, u32 lane) {
+  const u32 chunks = lane < ns ? (d.nbits + 31u) >> 5 : 0u;
+  const u32 incl = wave_scan(chunks);
+  if (lane < ns) tab[lane] = GroupSub{d.k0, d.nbits, d.lane, d.win, d.layout, incl - chunks, 0u};
+}
+
+// Work item i, loaded as two 16-byte words (a conditional copy of the struct would go
+// through scratch memory).
+__device__ __forceinline__ ItemDesc load_item(const ItemDesc *items, u32 i) {
+  const uint4 a = ((const uint4 *)(items + i))[0], b = ((const uint4 *)(items + i))[1];
+  ItemDesc d;
+  d.head.k0 = (u64)a.x | ((u64)a.y << 32);
+  d.head.nbits = a.z;
+  d.head.lane = a.w;
+  d.head.win = b.x;
+  d.head.layout = b.y;
+  d.first = b.z;
+  d.count = b.w;
+  return d;
+}
+
+// The sub-tile of a group's table (ns entries) whose chunks include chunk c.
+__device__ __forceinline__ u32 sub_of(const GroupSub *tab, u32 ns, u32 c) {
+  u32 lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const u32 mid = (lo + hi + 1u) >> 1;
+    if (tab[mid].c0 <= c) lo = mid;
+    else hi = mid - 1u;
+  }
+  return lo;
+}
+
+// Chunk c of the byte map from its 32 alive bits: one byte (0 or 1) per position.
+__device__ __forceinline__ void store_chunk(u32 *map32, u32 c, u32 word) {
+  auto bytes = [&](u32 q) { return (((word >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u; };
+  ((uint4 *)map32)[2 * c] = make_uint4(bytes(0), bytes(1), bytes(2), bytes(3));
+  ((uint4 *)map32)[2 * c + 1] = make_uint4(bytes(4), bytes(5), bytes(6), bytes(7));
+}
+
 template <bool kPersistent, bool kRowInLds>
 __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ tiles,
+                                                const ItemDesc *__restrict__ items,
                                                 const BatchHdr *__restrict__ hdr,
                                                 const LayoutDev *__restrict__ lays,
                                                 uint4 *__restrict__ tile_counts, u64 *__restrict__ surv_p,
                                                 u32 *__restrict__ surv_w, u32 cap,
                                                 u32 *__restrict__ surv_count, u32 *__restrict__ queue,
                                                 u32 lds_words, u32 off_words, u32 mask_words, u64 *stamp) {
-  extern __shared__ u32 lds[];
+  extern __shared__ __attribute__((aligned(16))) u32 lds[];
+  // A group's sub-tiles (two tables: the next group's is stored while the current one's
+  // compaction may still read its own) and their counters.
+  __shared__ GroupSub gsub[2][kMaxGroup];
+  __shared__ u32 gcnt[kMaxGroup][3];
   u32 *map32 = lds;
   u8 *map = (u8 *)lds;
   u16 *soff = (u16 *)(lds + lds_words);
   u32 *smask = lds + lds_words + off_words;
-  u32 *sc = smask + mask_words;   // [2] the tile after next
+  u32 *sc = smask + mask_words;   // [2] the item after next
   const u32 tid = threadIdx.x, bs = blockDim.x;
   const u32 lane64 = tid & 63u;
-  const u32 ntiles = hdr->ntiles;
+  const u32 nitems = hdr->nitems;
   const bool sieve = hdr->has_sieve != 0;
+  const u32 wave = tid >> 6, nwaves = (bs + 63u) >> 6;
   stamp_start(stamp);
-  // Persistent form: workgroup g takes tiles g and gridDim.x + g first, and the queue
-  // hands out tiles from 2 gridDim.x on (every workgroup claiming from one counter at
-  // the start would serialize on that address). A workgroup loads the next tile's
-  // descriptor at the top of the current one, claims the tile after next once the
+  // Persistent form: workgroup g takes items g and gridDim.x + g first, and the queue
+  // hands out items from 2 gridDim.x on (every workgroup claiming from one counter at
+  // the start would serialize on that address). A workgroup loads the next item's
+  // descriptor at the top of the current one, claims the item after next once the
   // claiming wave's marking loads are done (the wave then has no load to wait for until
-  // the end of the tile, so the claim's latency stays hidden) and, once the current tile's
-  // marking is done and its offset row is dead, loads the next tile's offset and mask
-  // rows into registers while it counts and compacts, storing them into LDS at the end of
-  // the tile. The claimer is lane 0 of the last wave.
+  // the end of the item, so the claim's latency stays hidden) and, once the current
+  // item's marking is done, loads what the next item needs into registers while it counts
+  // and compacts, storing it into LDS at the end of the item: a single tile's offset and
+  // mask rows, or a group's sub-tile table. The claimer is lane 0 of the last wave.
   const u32 claimer = (((bs + 63u) >> 6) - 1u) * 64u;
   u32 t = blockIdx.x, tnext = gridDim.x + blockIdx.x;
-  TileDesc td = t < ntiles ? tiles[t] : TileDesc{};
-  bool row_ready = false;   // the tile's offset and mask rows are in LDS already
-  for (;;) {
-    if (t >= ntiles) break;
+  ItemDesc it{};
+  if (t < nitems) it = load_item(items, t);
+  bool row_ready = false;   // a single tile's offset and mask rows are in LDS already
+  bool sub_ready = false;   // a group's sub-tile table is in LDS already
+  // Group counters start at zero (ordered before any count by the first item's staging
+  // barrier); each group's flush zeroes them again for the next.
+  if (tid < kMaxGroup) gcnt[tid][0] = gcnt[tid][1] = gcnt[tid][2] = 0u;
+  for (u32 iter = 0;; ++iter) {
+    if (t >= nitems) break;
     u32 claim = 0;
-    TileDesc tdn{};
-    if (kPersistent && tnext < ntiles) tdn = tiles[tnext];
-    const LayoutDev *L = lays + td.layout;
-    const u32 nb = td.nbits, nc = (nb + 31u) >> 5;
-    const u32 P = L->P;
-    const u16 *grow = L->off + (size_t)td.lane * L->ostride;
-    if (!row_ready) {
-      if (kRowInLds) {
-        const uint4 *src = (const uint4 *)grow;
-        uint4 *dst = (uint4 *)soff;
-        for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
+    ItemDesc itn{};
+    if (kPersistent && tnext < nitems) itn = load_item(items, tnext);
+    const u32 cur = iter & 1u;
+    const TileDesc td = it.head;
+    const bool single = it.count == 1;
+    const u32 ns = it.count;
+
+    // ---- staging at the item's top (the first item of a workgroup, or what could not be
+    // loaded ahead)
+    if (single) {
+      if (!row_ready) {
+        const LayoutDev *L = lays + td.layout;
+        if (kRowInLds) {
+          const uint4 *src = (const uint4 *)(L->off + (size_t)td.lane * L->ostride);
+          uint4 *dst = (uint4 *)soff;
+          for (u32 i = tid, nv = L->ostride >> 3; i < nv; i += bs) dst[i] = src[i];
+        }
+        const u32 *msrc = L->mask + (size_t)td.lane * L->mask_words;
+        for (u32 i = tid; i < L->mask_words; i += bs) smask[i] = msrc[i];
+        __syncthreads();
       }
-      const u32 *msrc = L->mask + (size_t)td.lane * L->mask_words;
-      for (u32 i = tid; i < L->mask_words; i += bs) smask[i] = msrc[i];
-      __syncthreads();
+    } else {
+      if (!sub_ready) {
+        if (tid < 64) {
+          TileDesc d{};
+          if (tid < ns) d = tiles[it.first + tid];
+          stage_sub(gsub[cur], d, ns, tid);
+        }
+        __syncthreads();
+      }
     }
 
     // ---- init: wheel mask and tail mask, expanded to one byte per position
     u32 my_pos = 0;
-    {
+    u32 nct;   // chunks of the item's map
+    if (single) {
+      const LayoutDev *L = lays + td.layout;
+      const u32 nb = td.nbits, P = L->P;
+      nct = (nb + 31u) >> 5;
       const u32 kP = P == 1 ? 0u : mod64(td.k0, P, L->Pmagic, L->Pc32);
-      for (u32 c = tid; c < nc; c += bs) {
+      for (u32 c = tid; c < nct; c += bs) {
         u32 word = 0xFFFFFFFFu;
         if (P != 1) {
           const u32 b = mod32(kP + 32u * c, P, L->Pmagic);
           word = __builtin_amdgcn_alignbit(smask[(b >> 5) + 1], smask[b >> 5], b & 31u);
         }
-        if (c == nc - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
+        if (c == nct - 1 && (nb & 31u)) word &= (1u << (nb & 31u)) - 1u;
         my_pos += __popc(word);
-        auto bytes = [&](u32 q) { return (((word >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u; };
-        ((uint4 *)map32)[2 * c] = make_uint4(bytes(0), bytes(1), bytes(2), bytes(3));
-        ((uint4 *)map32)[2 * c + 1] = make_uint4(bytes(4), bytes(5), bytes(6), bytes(7));
+        store_chunk(map32, c, word);
+      }
+    } else {
+      const GroupSub &last = gsub[cur][ns - 1];
+      nct = last.c0 + ((last.nb + 31u) >> 5);
+      for (u32 c = tid; c < nct; c += bs) {
+        const u32 j = sub_of(gsub[cur], ns, c);
+        const GroupSub &g = gsub[cur][j];
+        const LayoutDev *L = lays + g.layout;
+        const u32 P = L->P, cl = c - g.c0, ncj = (g.nb + 31u) >> 5;
+        u32 word = 0xFFFFFFFFu;
+        if (P != 1) {
+          const u32 kP = mod64(g.k0, P, L->Pmagic, L->Pc32);
+          const u32 b = mod32(kP + 32u * cl, P, L->Pmagic);
+          const u32 *mrow = L->mask + (size_t)g.lane * L->mask_words;
+          word = __builtin_amdgcn_alignbit(mrow[(b >> 5) + 1], mrow[b >> 5], b & 31u);
+        }
+        if (cl == ncj - 1 && (g.nb & 31u)) word &= (1u << (g.nb & 31u)) - 1u;
+        atomicAdd(&gcnt[j][0], (u32)__popc(word));
+        store_chunk(map32, c, word);
       }
     }
     __syncthreads();
-    // ---- stage B: certified filters. The offset row is read from LDS where it was
-    // staged (kRowInLds), else from global memory.
-    const bool wide = (td.k0 >> 32) != 0;
-    const u32 wave = tid >> 6, nwaves = (bs + 63u) >> 6;
-    auto mark = [&](u32 from, u32 to) {
-      if (kRowInLds) {
-        if (wide) mark_range<true>(map, nb, L, soff, td.k0, from, to, wave, nwaves, lane64);
-        else mark_range<false>(map, nb, L, soff, td.k0, from, to, wave, nwaves, lane64);
+
+    // ---- stage B: certified filters (a single tile reads its offset row from LDS where
+    // it was staged, a group reads its sub-tiles' rows from global memory)
+    auto mark = [&](bool covers) {
+      if (single) {
+        const LayoutDev *L = lays + td.layout;
+        const u32 from = covers ? 0u : L->ncov, to = covers ? L->ncov : L->nf;
+        const bool wide = (td.k0 >> 32) != 0;
+        const u16 *row = kRowInLds ? (const u16 *)soff : L->off + (size_t)td.lane * L->ostride;
+        if (wide) mark_range<true>(map, td.nbits, L, row, td.k0, from, to, wave, nwaves, lane64);
+        else mark_range<false>(map, td.nbits, L, row, td.k0, from, to, wave, nwaves, lane64);
       } else {
-        if (wide) mark_range<true>(map, nb, L, grow, td.k0, from, to, wave, nwaves, lane64);
-        else mark_range<false>(map, nb, L, grow, td.k0, from, to, wave, nwaves, lane64);
+        for (u32 j = 0; j < ns; ++j) {
+          const GroupSub &g = gsub[cur][j];
+          const LayoutDev *L = lays + g.layout;
+          const u32 from = covers ? 0u : L->ncov, to = covers ? L->ncov : L->nf;
+          const u16 *row = L->off + (size_t)g.lane * L->ostride;
+          if ((g.k0 >> 32) != 0) mark_range<true>(map + 32u * g.c0, g.nb, L, row, g.k0, from, to, wave, nwaves, lane64);
+          else mark_range<false>(map + 32u * g.c0, g.nb, L, row, g.k0, from, to, wave, nwaves, lane64);
+        }
       }
     };
-    mark(0, L->ncov);
+    mark(true);
     if (kPersistent && !sieve && tid == claimer) claim = atomicAdd(queue + opaque_zero(), 1u);
     __syncthreads();
     u32 my_after = 0;
     if (sieve) {
-      for (u32 c = tid; c < nc; c += bs) my_after += __popc(chunk_bits(map32, c));
+      for (u32 c = tid; c < nct; c += bs) {
+        const u32 n = __popc(chunk_bits(map32, c));
+        if (single) my_after += n;
+        else if (n) atomicAdd(&gcnt[sub_of(gsub[cur], ns, c)][1], n);
+      }
       // Every wave's count of the covers-only map completes before any wave's stage B'
       // store: a wave's chunks are not the bytes it marks.
       __syncthreads();
       // ---- stage B': small primes
-      mark(L->ncov, L->nf);
+      mark(false);
       if (kPersistent && tid == claimer) claim = atomicAdd(queue + opaque_zero(), 1u);
       __syncthreads();
     }
-    // ---- marking is over and the offset and mask rows dead: load the next tile's rows
-    // into registers now, store them into LDS after the compaction
+    // ---- marking is over and the rows dead: load what the next item needs into
+    // registers now, store it into LDS after the compaction
     u32 nv_next = 0, nm_next = 0, mpre = 0;
     uint4 pre0 = make_uint4(0, 0, 0, 0), pre1 = pre0, pre2 = pre0, pre3 = pre0;
+    TileDesc dnext{};
     if (kPersistent) {
       row_ready = false;
-      if (tnext < ntiles) {
+      sub_ready = false;
+      if (tnext < nitems && itn.count == 1) {
+        const TileDesc &tdn = itn.head;
         const LayoutDev *Ln = lays + tdn.layout;
         const u32 nv = kRowInLds ? Ln->ostride >> 3 : 0u, nm = Ln->mask_words;
         if (nv <= kRowPrefetch * bs && nm <= bs) {
@@ -341,21 +464,30 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
           nm_next = nm;
           row_ready = true;
         }
+      } else if (tnext < nitems) {
+        if (tid < itn.count) dnext = tiles[itn.first + tid];
+        sub_ready = true;
       }
     }
-    u32 my_surv = 0;
-    for (u32 c = tid; c < nc; c += bs) my_surv += __popc(chunk_bits(map32, c));
-    if (!sieve) my_after = my_surv;
 
     // ---- counters and survivors, wave by wave (inclusive survivor scan within the wave)
+    u32 my_surv = 0;
+    for (u32 c = tid; c < nct; c += bs) {
+      const u32 n = __popc(chunk_bits(map32, c));
+      my_surv += n;
+      if (!single && n) atomicAdd(&gcnt[sub_of(gsub[cur], ns, c)][2], n);
+    }
+    if (!sieve) my_after = my_surv;
     const u32 incl = wave_scan(my_surv);
     const u32 wsur = (u32)__builtin_amdgcn_readlane((int)incl, 63);
-    const u32 wpos = wave_sum(my_pos), waft = sieve ? wave_sum(my_after) : wsur;
-    if (lane64 == 0) {
-      u32 *slot = (u32 *)(tile_counts + t);
-      atomicAdd(slot, wpos);
-      if (waft) atomicAdd(slot + 1, waft);
-      if (wsur) atomicAdd(slot + 2, wsur);
+    if (single) {
+      const u32 wpos = wave_sum(my_pos), waft = sieve ? wave_sum(my_after) : wsur;
+      if (lane64 == 0) {
+        u32 *slot = (u32 *)(tile_counts + it.first);
+        atomicAdd(slot, wpos);
+        if (waft) atomicAdd(slot + 1, waft);
+        if (wsur) atomicAdd(slot + 2, wsur);
+      }
     }
     if (wsur) {
       u32 base = 0;
@@ -363,19 +495,47 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       base = (u32)__builtin_amdgcn_readfirstlane((int)base);
       if (my_surv) {
         u32 idx = base + incl - my_surv;
-        const u64 c = L->lanes[td.lane], Mp = L->Mp;
-        for (u32 ch = tid; ch < nc; ch += bs) {
+        for (u32 ch = tid; ch < nct; ch += bs) {
           u32 x = chunk_bits(map32, ch);
+          if (!x) continue;
+          u64 c, Mp, k;
+          u32 win;
+          if (single) {
+            const LayoutDev *L = lays + td.layout;
+            c = L->lanes[td.lane];
+            Mp = L->Mp;
+            k = td.k0 + 32ull * ch;
+            win = td.win;
+          } else {
+            const GroupSub &g = gsub[cur][sub_of(gsub[cur], ns, ch)];
+            const LayoutDev *L = lays + g.layout;
+            c = L->lanes[g.lane];
+            Mp = L->Mp;
+            k = g.k0 + 32ull * (ch - g.c0);
+            win = g.win;
+          }
           while (x) {
             const u32 i = __builtin_ctz(x);
             x &= x - 1;
             if (idx < cap) {
-              surv_p[idx] = c + Mp * (td.k0 + 32ull * ch + i);
-              surv_w[idx] = td.win;
+              surv_p[idx] = c + Mp * (k + i);
+              surv_w[idx] = win;
             }
             ++idx;
           }
         }
+      }
+    }
+    if (!single) {
+      // A group's counters are complete once every wave has counted: one store per
+      // sub-tile (k_stage zeroed the slots, and each sub-tile belongs to one item). The
+      // thread that reads a slot zeroes it; the item's last barrier orders that before the
+      // next group's counts.
+      __syncthreads();
+      if (tid < kMaxGroup) {
+        const u32 pos = gcnt[tid][0], aft = gcnt[tid][1], sur = gcnt[tid][2];
+        if (tid < ns) tile_counts[it.first + tid] = make_uint4(pos, sieve ? aft : sur, sur, 0u);
+        gcnt[tid][0] = gcnt[tid][1] = gcnt[tid][2] = 0u;
       }
     }
     if (!kPersistent) break;
@@ -387,26 +547,28 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
       if (tid + 3 * bs < nv_next) dst[tid + 3 * bs] = pre3;
       if (tid < nm_next) smask[tid] = mpre;
     }
+    if (sub_ready && tid < 64) stage_sub(gsub[cur ^ 1u], dnext, itn.count, tid);
     if (tid == claimer) sc[2] = 2 * gridDim.x + claim;
-    __syncthreads();   // the next tile reuses the map and the scalars, and finds its rows staged
+    __syncthreads();   // the next item reuses the map and the scalars, and finds its rows or table staged
     t = tnext;
-    td = tdn;
+    it = itn;
     tnext = sc[2];
   }
-  // Every wave leaves the loop at the same barrier (persistent) or after the tile's last
+  // Every wave leaves the loop at the same barrier (persistent) or after the item's last
   // barrier, so thread 0's time is the workgroup's.
   stamp_end(stamp, threadIdx.x == 0);
 }
 
 // Stages one batch: zeroes the survivor count, the queue heads and the tile counters,
-// and copies the batch header and the tile descriptors from pinned host memory (written
+// and copies the batch header, the tile descriptors and k_sieve's work items from pinned host memory (written
 // by the host before the launch) into device memory. A kernel rather than a memset and two copies, so a
 // captured graph holds no memset or copy node and no copy engine has to be waited for
 // between batches. The tile count comes from the header itself, so one captured graph
 // serves every batch that fits the buffers; the host keeps ntiles within the buffers'
 // capacity. Every other per-batch output is written in full by the kernels.
 __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restrict__ h_tiles,
-                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, uint4 *__restrict__ tile_counts,
+                        BatchHdr *__restrict__ hdr, uint2 *__restrict__ tiles, const uint4 *__restrict__ h_items,
+                        uint4 *__restrict__ items, uint4 *__restrict__ tile_counts,
                         u32 *__restrict__ scalars, u64 *__restrict__ board_words, u32 nboard_words,
                         u64 *__restrict__ h_sync, u64 *__restrict__ stamps, u32 *__restrict__ active) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
@@ -424,6 +586,7 @@ __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restr
   for (u32 i = g; i < nboard_words; i += stride) board_words[i] = 0;   // stage D's board of posted splits
   const u32 n = h.ntiles * (u32)(sizeof(TileDesc) / sizeof(uint2));
   for (u32 i = g; i < n; i += stride) tiles[i] = h_tiles[i];
+  for (u32 i = g; i < 2 * h.nitems; i += stride) items[i] = h_items[i];
   for (u32 i = g; i < h.ntiles; i += stride) tile_counts[i] = make_uint4(0, 0, 0, 0);   // k_sieve adds per wave
 }
 
@@ -1652,7 +1815,9 @@ struct Slot {
   size_t graph_lds = 0;
   bool graph_tail = false;
   TileDesc *d_tiles = nullptr;
+  dev::ItemDesc *d_items = nullptr;   // k_sieve's work items (at most one per tile)
   size_t tiles_cap = 0;
+  size_t ni = 0;                      // work items of the batch
   dev::BatchHdr *d_hdr = nullptr;
   dev::BatchHdr *h_hdr = nullptr;
   uint4 *d_tile_counts = nullptr;   // per tile {positions, alive after covers, survivors, 0}
@@ -1675,6 +1840,8 @@ struct Slot {
   u32 seq = 0;                      // number of the batch in flight in this slot
   TileDesc *h_tiles = nullptr;
   TileDesc *dh_tiles = nullptr;
+  dev::ItemDesc *h_items = nullptr;
+  dev::ItemDesc *dh_items = nullptr;
   dev::BatchHdr *dh_hdr = nullptr;
   uint4 *h_tile_counts = nullptr;
   u64 *h_surv_p = nullptr;
@@ -1741,11 +1908,11 @@ DeviceCtx::~DeviceCtx() {
                     (void *)m.d_active})
       if (p) (void)hipFree(p);
     for (Slot &b : m.slot) {
-      for (void *p : {(void *)b.d_tiles, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
+      for (void *p : {(void *)b.d_tiles, (void *)b.d_items, (void *)b.d_hdr, (void *)b.d_tile_counts, (void *)b.d_surv_p,
                       (void *)b.d_surv_w, (void *)b.d_flags, (void *)b.d_one, (void *)b.d_rung,
                       (void *)b.d_scalars, (void *)b.d_board, (void *)b.d_stamps})
         if (p) (void)hipFree(p);
-      for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_sync, (void *)b.h_tiles, (void *)b.h_tile_counts,
+      for (void *p : {(void *)b.h_hdr, (void *)b.h_count, (void *)b.h_sync, (void *)b.h_tiles, (void *)b.h_items, (void *)b.h_tile_counts,
                       (void *)b.h_surv_p, (void *)b.h_surv_w, (void *)b.h_flags, (void *)b.h_rung})
         if (p) (void)hipHostFree(p);
       if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
@@ -1787,7 +1954,11 @@ int DeviceCtx::open(int device) {
   hipDeviceProp_t p;
   HIP_TRY(hipGetDeviceProperties(&p, device));
   cus_ = p.multiProcessorCount;
-  lds_limit_ = p.sharedMemPerBlock;
+  // The dynamic LDS a k_sieve workgroup may take: the device's per-workgroup LDS less the
+  // kernel's static arrays (a group's sub-tile tables and counters).
+  hipFuncAttributes fa{};
+  HIP_TRY(hipFuncGetAttributes(&fa, (const void *)dev::k_sieve<true, true>));
+  lds_limit_ = p.sharedMemPerBlock > fa.sharedSizeBytes ? p.sharedMemPerBlock - fa.sharedSizeBytes : 0;
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   clock_ms_per_tick_ = khz > 0 ? 1.0 / khz : 1e-5;
@@ -1958,7 +2129,8 @@ int DeviceCtx::enqueue_batch(int si) {
   const LaunchCfg &cfg = b.cfg;
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
     dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
-                                                        (uint2 *)b.d_tiles, b.d_tile_counts, b.d_scalars,
+                                                        (uint2 *)b.d_tiles, (const uint4 *)b.dh_items,
+                                                        (uint4 *)b.d_items, b.d_tile_counts, b.d_scalars,
                                                         (u64 *)b.d_board, dev::kBoardSlots * 4u, b.dh_sync,
                                                         b.d_stamps, m.d_active);
     HIP_TRY(hipGetLastError());
@@ -1967,13 +2139,13 @@ int DeviceCtx::enqueue_batch(int si) {
     const hipEvent_t e1 = timed ? b.ev1 : nullptr, e2 = timed ? b.ev2 : nullptr;
     if (cfg.persistent) {
       hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<true, true> : dev::k_sieve<true, false>, dim3(b.grid), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
-                            (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
+                            (const TileDesc *)b.d_tiles, (const dev::ItemDesc *)b.d_items, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
                             (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, m.mask_max,
                             b.d_stamps);
     } else {
-      hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<false, true> : dev::k_sieve<false, false>, dim3((u32)b.nt), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
-                            (const TileDesc *)b.d_tiles, (const dev::BatchHdr *)b.d_hdr,
+      hipExtLaunchKernelGGL(b.off_words ? dev::k_sieve<false, true> : dev::k_sieve<false, false>, dim3((u32)b.ni), dim3(cfg.block), (u32)b.lds, st, e1, e2, 0,
+                            (const TileDesc *)b.d_tiles, (const dev::ItemDesc *)b.d_items, (const dev::BatchHdr *)b.d_hdr,
                             (const dev::LayoutDev *)m.d_lays, b.d_tile_counts, b.d_surv_p, b.d_surv_w,
                             (u32)b.surv_cap, b.d_scalars, b.d_scalars + 1, b.lds_words, b.off_words, m.mask_max,
                             b.d_stamps);
@@ -2058,7 +2230,7 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
   HIP_TRY(hipSetDevice(device_));
   const size_t nt = tiles.size();
   if (cfg.block == 0 || cfg.block > 1024 || cfg.block % 64 || cfg.max_bits == 0 || cfg.max_bits % 32 ||
-      cfg.max_bits > kMaxTileBits) {
+      cfg.max_bits > kMaxTileBits || cfg.group == 0 || cfg.group > dev::kMaxGroup) {
     err_ = "bad launch configuration";
     return -1;
   }
@@ -2081,9 +2253,12 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
       if (b.d_tiles) HIP_TRY(hipFree(b.d_tiles));
       b.tiles_cap = std::max<size_t>(nt, 2 * b.tiles_cap);
       HIP_TRY(hipMalloc(&b.d_tiles, b.tiles_cap * sizeof(TileDesc)));
+      if (b.d_items) HIP_TRY(hipFree(b.d_items));
+      HIP_TRY(hipMalloc(&b.d_items, b.tiles_cap * sizeof(dev::ItemDesc)));
       if (b.d_tile_counts) HIP_TRY(hipFree(b.d_tile_counts));
       HIP_TRY(hipMalloc(&b.d_tile_counts, b.tiles_cap * sizeof(uint4)));
       HIP_TRY(pinned_alloc((void **)&b.h_tiles, (void **)&b.dh_tiles, b.tiles_cap * sizeof(TileDesc)));
+      HIP_TRY(pinned_alloc((void **)&b.h_items, (void **)&b.dh_items, b.tiles_cap * sizeof(dev::ItemDesc)));
       HIP_TRY(pinned_alloc((void **)&b.h_tile_counts, (void **)&b.dh_tile_counts, b.tiles_cap * sizeof(uint4)));
       b.graph_ok = false;
     }
@@ -2095,11 +2270,30 @@ int DeviceCtx::launch(const std::vector<TileDesc> &tiles, u32 nwin, bool has_sie
     b.h_hdr->seq = ++b.seq;
     std::memcpy(b.h_tiles, tiles.data(), nt * sizeof(TileDesc));   // k_stage copies both in
 
-    // One LDS byte per position, sized to the batch's largest tile (rounded to whole
-    // 32-position chunks): short tiles then leave room for more resident workgroups.
+    // Work items: a tile of at least half the largest tile size is an item alone; shorter
+    // consecutive tiles (the planner sends them out longest first, so they come last) are
+    // grouped, up to cfg.group tiles and max_bits positions (each from a 32-position chunk
+    // boundary). A group shares one init, one count and the barriers, but marks its tiles
+    // one after another with their offset rows read from global memory, so long groups
+    // lose what they save (the 512-window batch from 1e9 to 1.8e19: 97.3 us one tile per
+    // item, 77.4 in groups of up to 4, 99.6 of up to 16). The map is sized to the largest
+    // item (whole chunks): short items then leave room for more resident workgroups.
+    auto padded = [](u32 nb) { return (nb + 31u) & ~31u; };
+    size_t ni = 0;
     u32 tile_max = 0;
-    for (const TileDesc &t : tiles) tile_max = std::max(tile_max, t.nbits);
-    const u32 lds_bytes = (tile_max + 31u) & ~31u;
+    for (size_t i = 0; i < nt;) {
+      size_t j = i + 1;
+      u32 bits = padded(tiles[i].nbits);
+      if (2 * tiles[i].nbits < cfg.max_bits)
+        while (j < nt && j - i < cfg.group && bits + padded(tiles[j].nbits) <= cfg.max_bits)
+          bits += padded(tiles[j++].nbits);
+      b.h_items[ni++] = dev::ItemDesc{tiles[i], (u32)i, (u32)(j - i)};
+      tile_max = std::max(tile_max, bits);
+      i = j;
+    }
+    b.ni = ni;
+    b.h_hdr->nitems = (u32)ni;
+    const u32 lds_bytes = tile_max;
     b.lds_words = lds_bytes / 4;
     // The tile's row of filter offsets is staged in LDS beside the map when both fit.
     // Large generated filter sets (about 77,000 marking units at mmax 10000, a 154 KB

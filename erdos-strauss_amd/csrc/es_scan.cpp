@@ -262,15 +262,17 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     double best_ms = 1e300;
     for (u32 block : {128u, 256u, 512u, 1024u})
       for (u32 bits : {8192u, 16384u, 32768u})
-        for (u32 waves : {16u, 24u, 0u}) {
+        for (u32 waves : {16u, 24u, 0u})
+          for (u32 group : {1u, 2u, 4u}) {
           if (waves && waves < block / 64) continue;
           tune_.launch.block = block;
           tune_.launch.max_bits = bits;
           tune_.launch.waves_per_cu = waves;
+          tune_.launch.group = group;
           double ms;
           if (time_batch(sample, ms)) { tune_ = saved; return -1; }
-          snprintf(line, sizeof line, "launch block=%u tile_bits=%u waves_per_cu=%u sieve_ms=%.4f\n", block, bits,
-                   waves, ms);
+          snprintf(line, sizeof line, "launch block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n", block,
+                   bits, waves, group, ms);
           report += line;
           if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
         }
@@ -303,14 +305,75 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
       }
       tune_.table[i].second = best;
     }
+    // Those rows were timed on batches of one window size, and for the larger sizes a few
+    // windows make too few tiles to fill the grid, so layouts of many short lanes win there
+    // on parallelism alone. In a batch of many windows the total work decides instead (the
+    // 512-window batch from 1e9 to 1.8e19: wheel 840 for its 2^26 windows runs the sieve in
+    // 83 us against 87 with 10920). So every row the sample uses is tried on the sample
+    // itself with each layout (one pass, rows in order), keeping what is faster there, and
+    // the launch shape is chosen again for the layouts that result.
+    double cur_ms;
+    if (time_batch(sample, cur_ms)) { tune_ = saved; return -1; }
+    auto row_of = [&](u64 w) {
+      const double lw = std::log2((double)std::max<u64>(w, 1));
+      size_t r = 0;
+      for (size_t k = 1; k < tune_.table.size(); ++k)
+        if (std::fabs(tune_.table[k].first - lw) < std::fabs(tune_.table[r].first - lw)) r = k;
+      return r;
+    };
+    std::vector<bool> used(tune_.table.size(), false);
+    for (const Interval &w : sample) if (w.hi > w.lo) used[row_of(w.hi - w.lo)] = true;
+    bool changed = false;
+    for (size_t i = 0; i < tune_.table.size(); ++i) {
+      if (!used[i]) continue;
+      u64 keep = tune_.table[i].second;
+      for (const Estimate &e : est_) {
+        if (e.Mp == keep) continue;
+        const u64 was = tune_.table[i].second;
+        tune_.table[i].second = e.Mp;
+        double ms;
+        if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+        if (ms < cur_ms) {
+          cur_ms = ms;
+          keep = e.Mp;
+        }
+        tune_.table[i].second = was;
+      }
+      if (keep != tune_.table[i].second) {
+        snprintf(line, sizeof line, "sample W=2^%.0f Mp=%llu sieve_ms=%.4f\n", tune_.table[i].first,
+                 (unsigned long long)keep, cur_ms);
+        report += line;
+        tune_.table[i].second = keep;
+        changed = true;
+      }
+    }
+    if (changed) {
+      for (u32 block : {128u, 256u, 512u, 1024u})
+        for (u32 bits : {8192u, 16384u, 32768u})
+          for (u32 waves : {16u, 24u, 0u})
+            for (u32 group : {1u, 2u, 4u}) {
+            if (waves && waves < block / 64) continue;
+            tune_.launch.block = block;
+            tune_.launch.max_bits = bits;
+            tune_.launch.waves_per_cu = waves;
+            tune_.launch.group = group;
+            double ms;
+            if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+            snprintf(line, sizeof line, "relaunch block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n",
+                     block, bits, waves, group, ms);
+            report += line;
+            if (ms < cur_ms) { cur_ms = ms; best_cfg = tune_.launch; }
+          }
+      tune_.launch = best_cfg;
+    }
   }
   tune_.launch.graph = saved.launch.graph;
   for (const auto &e : tune_.table) {
     snprintf(line, sizeof line, "choice W=2^%.0f Mp=%llu\n", e.first, (unsigned long long)e.second);
     report += line;
   }
-  snprintf(line, sizeof line, "choice block=%u tile_bits=%u waves_per_cu=%u\n", tune_.launch.block,
-           tune_.launch.max_bits, tune_.launch.waves_per_cu);
+  snprintf(line, sizeof line, "choice block=%u tile_bits=%u waves_per_cu=%u group=%u\n", tune_.launch.block,
+           tune_.launch.max_bits, tune_.launch.waves_per_cu, tune_.launch.group);
   report += line;
   return 0;
 }

@@ -108,6 +108,7 @@ def lib():
             "es_set_layout": (ctypes.c_int, [ctypes.c_void_p, U64]),
             "es_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]),
             "es_set_residency": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+            "es_set_grouping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
             "es_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "es_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
             "es_set_device_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -318,6 +319,9 @@ class Scanner:
 
     def set_residency(self, waves_per_cu=16):
         _check(lib().es_set_residency(self.h, waves_per_cu))
+
+    def set_grouping(self, max_tiles=4):
+        _check(lib().es_set_grouping(self.h, max_tiles))
 
     def set_graph(self, on=True):
         _check(lib().es_set_graph(self.h, 1 if on else 0))

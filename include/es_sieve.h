@@ -184,6 +184,13 @@ int es_set_launch(es_scanner *s, int persistent, uint32_t block, uint32_t max_bi
  * output does not depend on it. */
 int es_set_residency(es_scanner *s, uint32_t waves_per_cu);
 
+/* Most tiles one work item of the marking kernel groups (1 to 16, default 4; 1 = every
+ * tile alone). Tiles shorter than half the tile size are grouped, up to this many and one
+ * tile's positions, so they share one pass of the per-tile init, counts and barriers.
+ * autotune sweeps it. No counterpart in the reference (one thread block per segment,
+ * sieve/cuda/scan.cu:212-221); output does not depend on it. */
+int es_set_grouping(es_scanner *s, uint32_t max_tiles);
+
 /* Replays the persistent form's device pipeline (counter reset, marking, Miller-Rabin)
  * from one captured hipGraph (1) or launches it kernel by kernel (0, the default).
  * No counterpart in the reference, which launches per round (sieve/cuda/scan.cu:212-221). */

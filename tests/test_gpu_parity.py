@@ -322,6 +322,31 @@ def test_batch_equals_alone():
         assert rec == want
 
 
+@pytest.mark.parametrize("persistent,sieve,group", [(True, 0, 4), (True, 1000, 2), (False, 1000, 16), (True, 0, 16)])
+def test_grouped_items_equal_single_tiles(persistent, sieve, group):
+    """Short tiles grouped into shared work items give the records of one item per tile,
+    and the oracle's: windows of 2^10 to 2^22 integers from 1e9 (32-bit tile starts) to
+    1e16 (64-bit), with and without stage B', persistent or not, groups of 2 to 16."""
+    kw = dict(wheel=2042040, mmax=10000, sieve=sieve, filters=F204)
+    wins = []
+    for i, lo in enumerate((10 ** 9 + 7, 10 ** 12 + 11, 10 ** 13 + 5, 10 ** 16 + 3)):
+        for e in (10, 13, 16, 19, 22):
+            base = lo + i * 2 ** 23 + e * 2 ** 23
+            wins.append((base, base + 2 ** e + 17 * e))
+    sc = scanner(**kw)
+    sc.set_launch(persistent, 512, 32768)
+    print("one tile per item", persistent, sieve, flush=True)
+    sc.set_grouping(1)
+    alone, codes0 = sc.records(wins)
+    print("groups of up to", group, flush=True)
+    sc.set_grouping(group)
+    packed, codes1 = sc.records(wins)
+    assert packed == alone and codes1 == codes0
+    for (lo, hi), rec in zip(wins, packed):
+        want, _ = oracle_record(lo, hi, kw)
+        assert rec == want, (lo, hi)
+
+
 # ---------------------------------------------------------------- stage D on the device
 
 GOLDEN_LINE_FILES = ["w19_certificates.txt", "w64_head.txt", "w64_tail.txt", "s0_head.txt", "s0_tail.txt",

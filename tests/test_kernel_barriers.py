@@ -20,28 +20,37 @@ SRC = os.path.join(paths.PKG, "csrc", "es_device.hip")
 SIEVE_PHASES = [
     ("rows staged", "dst[i] = src[i];"),
     ("mask row staged", "smask[i] = msrc[i];"),
-    ("mask init", "((uint4 *)map32)[2 * c] = make_uint4(bytes(0)"),
-    ("stage B", "mark(0, L->ncov);"),
-    ("count after covers", "my_after += __popc(chunk_bits(map32, c));"),
-    ("stage B'", "mark(L->ncov, L->nf);"),
-    ("survivor count", "my_surv += __popc(chunk_bits(map32, c));"),
+    ("group table staged", "stage_sub(gsub[cur], d, ns, tid);"),
+    ("mask init", "store_chunk(map32, c, word);"),
+    ("group init", "atomicAdd(&gcnt[j][0], (u32)__popc(word));"),
+    ("stage B", "mark(true);"),
+    ("count after covers", "const u32 n = __popc(chunk_bits(map32, c));"),
+    ("stage B'", "mark(false);"),
+    ("survivor count", "my_surv += n;"),
     ("compaction", "u32 x = chunk_bits(map32, ch);"),
+    ("group counters flushed", "const u32 pos = gcnt[tid][0], aft = gcnt[tid][1], sur = gcnt[tid][2];"),
     ("next rows stored", "if (tid < nv_next) dst[tid] = pre0;"),
+    ("next group table stored", "stage_sub(gsub[cur ^ 1u], dnext, itn.count, tid);"),
     ("claim stored", "sc[2] = 2 * gridDim.x + claim;"),
     ("claim read", "tnext = sc[2];"),
-    ("next tile", "stamp_end(stamp, threadIdx.x == 0);"),   # after the loop: the next tile starts over
+    ("next tile", "stamp_end(stamp, threadIdx.x == 0);"),   # after the loop: the next item starts over
 ]
 # Ordered pairs (writer phase, the next phase that conflicts with it) and the LDS they share.
 SIEVE_ORDER = [
     ("mask row staged", "mask init"),            # the mask row (and the offset row) of a tile staged at its top
+    ("group table staged", "group init"),        # gsub[cur]: a group's sub-tile table staged at its top
     ("mask init", "stage B"),                    # the byte map
+    ("group init", "stage B"),                   # the byte map (a group's)
     ("stage B", "count after covers"),           # the byte map
     ("count after covers", "stage B'"),          # the byte map (counted before B' clears it)
     ("stage B'", "survivor count"),              # the byte map
-    ("claim stored", "claim read"),              # sc[2]: the tile after next
+    ("survivor count", "group counters flushed"),  # gcnt: every wave's counts before the flush reads them
+    ("claim stored", "claim read"),              # sc[2]: the item after next
     ("stage B'", "next rows stored"),            # the offset and mask rows: read by init and marking, then replaced
     ("next rows stored", "next tile"),           # the next tile's rows, read by its init and marking
-    ("compaction", "next tile"),                 # the map and sc[2], reused by the next tile
+    ("next group table stored", "next tile"),    # gsub[cur ^ 1]: the next group's table, read from its init on
+    ("group counters flushed", "next tile"),     # gcnt zeroed by the flush, counted into by the next group
+    ("compaction", "next tile"),                 # the map and sc[2], reused by the next item
 ]
 
 RUNG_PHASES = [
@@ -120,8 +129,8 @@ def test_k_rung_every_lds_phase_pair_has_a_barrier(src):
 
 def test_checker_finds_the_barrier_between_count_and_stage_b_prime(src):
     body = strip_comments(kernel_body(src, "k_sieve"))
-    a = body.index("my_after += __popc(chunk_bits(map32, c));")
-    b = body.index("mark(L->ncov, L->nf);")
+    a = body.index("const u32 n = __popc(chunk_bits(map32, c));")
+    b = body.index("mark(false);")
     cut = body[:a] + body[a:b].replace("__syncthreads();", "", 1) + body[b:]
     assert ("count after covers", "stage B'") in missing_barriers(cut, SIEVE_PHASES, SIEVE_ORDER)
 
