@@ -228,6 +228,50 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     sieve_ms = t[2];
     return 0;
   };
+  // Every launch shape on the sample, then the closest few timed again in interleaved
+  // rounds: single medians of five passes differ by a few percent from one call to the
+  // next (clocks, the device's other work), enough to rank close shapes the wrong way.
+  // Interleaving spreads any drift over all the finalists alike.
+  auto sweep_launch = [&](const char *tag, LaunchCfg &best_cfg, double &best_ms) -> int {
+    std::vector<std::pair<double, LaunchCfg>> timed;
+    for (u32 block : {128u, 256u, 512u, 1024u})
+      for (u32 bits : {8192u, 16384u, 32768u})
+        for (u32 waves : {16u, 24u, 0u})
+          for (u32 group : {1u, 2u, 4u}) {
+            if (waves && waves < block / 64) continue;
+            tune_.launch.block = block;
+            tune_.launch.max_bits = bits;
+            tune_.launch.waves_per_cu = waves;
+            tune_.launch.group = group;
+            double ms;
+            if (time_batch(sample, ms)) return -1;
+            snprintf(line, sizeof line, "%s block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n", tag,
+                     block, bits, waves, group, ms);
+            report += line;
+            timed.emplace_back(ms, tune_.launch);
+          }
+    std::sort(timed.begin(), timed.end(), [](const std::pair<double, LaunchCfg> &a,
+                                             const std::pair<double, LaunchCfg> &b) { return a.first < b.first; });
+    const size_t nf = std::min<size_t>(4, timed.size());
+    std::vector<std::vector<double>> again(nf);
+    for (int round = 0; round < 5; ++round)
+      for (size_t i = 0; i < nf; ++i) {
+        tune_.launch = timed[i].second;
+        double ms;
+        if (time_batch(sample, ms)) return -1;
+        again[i].push_back(ms);
+      }
+    for (size_t i = 0; i < nf; ++i) {
+      std::sort(again[i].begin(), again[i].end());
+      const double ms = again[i][2];
+      const LaunchCfg &c = timed[i].second;
+      snprintf(line, sizeof line, "%s-final block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n", tag,
+               c.block, c.max_bits, c.waves_per_cu, c.group, ms);
+      report += line;
+      if (ms < best_ms) { best_ms = ms; best_cfg = c; }
+    }
+    return 0;
+  };
   tune_.table.clear();
   tune_.launch.graph = false;
   std::vector<std::pair<double, u64>> table;
@@ -260,22 +304,7 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
   if (!sample.empty()) {
     LaunchCfg best_cfg = tune_.launch;
     double best_ms = 1e300;
-    for (u32 block : {128u, 256u, 512u, 1024u})
-      for (u32 bits : {8192u, 16384u, 32768u})
-        for (u32 waves : {16u, 24u, 0u})
-          for (u32 group : {1u, 2u, 4u}) {
-          if (waves && waves < block / 64) continue;
-          tune_.launch.block = block;
-          tune_.launch.max_bits = bits;
-          tune_.launch.waves_per_cu = waves;
-          tune_.launch.group = group;
-          double ms;
-          if (time_batch(sample, ms)) { tune_ = saved; return -1; }
-          snprintf(line, sizeof line, "launch block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n", block,
-                   bits, waves, group, ms);
-          report += line;
-          if (ms < best_ms) { best_ms = ms; best_cfg = tune_.launch; }
-        }
+    if (sweep_launch("launch", best_cfg, best_ms)) { tune_ = saved; return -1; }
     tune_.launch = best_cfg;
     // The layouts above were timed at the default launch shape, and close ones can rank
     // the other way at the tuned shape (one 2^34 window at 10^15: wheel 2042040 is 14%
@@ -310,8 +339,9 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     // on parallelism alone. In a batch of many windows the total work decides instead (the
     // 512-window batch from 1e9 to 1.8e19: wheel 840 for its 2^26 windows runs the sieve in
     // 83 us against 87 with 10920). So every row the sample uses is tried on the sample
-    // itself with each layout (one pass, rows in order), keeping what is faster there, and
-    // the launch shape is chosen again for the layouts that result.
+    // itself with each layout (rows in order; the two fastest of a row timed again in
+    // interleaved rounds), keeping what is faster there, and the launch shape is chosen
+    // again for the layouts that result.
     double cur_ms;
     if (time_batch(sample, cur_ms)) { tune_ = saved; return -1; }
     auto row_of = [&](u64 w) {
@@ -326,45 +356,48 @@ int Scanner::autotune(const std::vector<Interval> &sample, std::string &report, 
     bool changed = false;
     for (size_t i = 0; i < tune_.table.size(); ++i) {
       if (!used[i]) continue;
-      u64 keep = tune_.table[i].second;
+      const u64 was = tune_.table[i].second;
+      std::vector<std::pair<double, u64>> cand{{cur_ms, was}};
       for (const Estimate &e : est_) {
-        if (e.Mp == keep) continue;
-        const u64 was = tune_.table[i].second;
+        if (e.Mp == was) continue;
         tune_.table[i].second = e.Mp;
         double ms;
         if (time_batch(sample, ms)) { tune_ = saved; return -1; }
-        if (ms < cur_ms) {
-          cur_ms = ms;
-          keep = e.Mp;
-        }
-        tune_.table[i].second = was;
+        cand.emplace_back(ms, e.Mp);
       }
-      if (keep != tune_.table[i].second) {
+      std::sort(cand.begin(), cand.end());
+      u64 keep = cand[0].second;
+      double keep_ms = cand[0].first;
+      if (cand.size() > 1) {   // the two fastest again, in interleaved rounds
+        std::vector<double> t0, t1;
+        for (int round = 0; round < 3; ++round) {
+          double ms;
+          tune_.table[i].second = cand[0].second;
+          if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+          t0.push_back(ms);
+          tune_.table[i].second = cand[1].second;
+          if (time_batch(sample, ms)) { tune_ = saved; return -1; }
+          t1.push_back(ms);
+        }
+        std::sort(t0.begin(), t0.end());
+        std::sort(t1.begin(), t1.end());
+        if (t1[1] < t0[1]) { keep = cand[1].second; keep_ms = t1[1]; }
+        else keep_ms = t0[1];
+      }
+      tune_.table[i].second = keep;
+      cur_ms = keep_ms;
+      if (keep != was) {
         snprintf(line, sizeof line, "sample W=2^%.0f Mp=%llu sieve_ms=%.4f\n", tune_.table[i].first,
                  (unsigned long long)keep, cur_ms);
         report += line;
-        tune_.table[i].second = keep;
         changed = true;
       }
     }
     if (changed) {
-      for (u32 block : {128u, 256u, 512u, 1024u})
-        for (u32 bits : {8192u, 16384u, 32768u})
-          for (u32 waves : {16u, 24u, 0u})
-            for (u32 group : {1u, 2u, 4u}) {
-            if (waves && waves < block / 64) continue;
-            tune_.launch.block = block;
-            tune_.launch.max_bits = bits;
-            tune_.launch.waves_per_cu = waves;
-            tune_.launch.group = group;
-            double ms;
-            if (time_batch(sample, ms)) { tune_ = saved; return -1; }
-            snprintf(line, sizeof line, "relaunch block=%u tile_bits=%u waves_per_cu=%u group=%u sieve_ms=%.4f\n",
-                     block, bits, waves, group, ms);
-            report += line;
-            if (ms < cur_ms) { cur_ms = ms; best_cfg = tune_.launch; }
-          }
-      tune_.launch = best_cfg;
+      LaunchCfg re_cfg = tune_.launch;
+      double re_ms = 1e300;
+      if (sweep_launch("relaunch", re_cfg, re_ms)) { tune_ = saved; return -1; }
+      tune_.launch = re_cfg;
     }
   }
   tune_.launch.graph = saved.launch.graph;
