@@ -23,6 +23,19 @@ int fail(int code, const std::string &m) {
   g_err = m;
   return code;
 }
+
+// The ES_MAX_IN_FLIGHT batch slots overlap on the device only if each slot's stream has a
+// hardware queue of its own. HIP gives a process GPU_MAX_HW_QUEUES queues (four by default)
+// and streams beyond that share them in order, so a batch waits behind the stage-D kernel of
+// the batch four slots back (the 512-window batch eight deep: 0.125 ms per batch with four
+// queues, 0.113 with eight or sixteen). The library raises the setting to the slot count
+// when it is lower, when it is loaded: the HIP runtime reads it at its first call in the
+// process, so a process that used HIP before loading the library keeps what it had.
+// Priority 101 runs this before the constructors that register the device code.
+__attribute__((constructor(101))) void raise_hw_queues() {
+  const char *v = std::getenv("GPU_MAX_HW_QUEUES");
+  if (!v || std::atoi(v) < ES_MAX_IN_FLIGHT) setenv("GPU_MAX_HW_QUEUES", std::to_string(ES_MAX_IN_FLIGHT).c_str(), 1);
+}
 }  // namespace
 
 extern "C" {
