@@ -145,3 +145,19 @@ def test_capi_refuses_empty_windows_outside_the_range():
         assert sc.check_interval(10 ** 13, 10 ** 13) and sc.check_interval(top, top)
         with pytest.raises(es.EsError, match="device"):
             sc.records([(10 ** 13, 10 ** 13)])
+
+
+@pytest.mark.parametrize("given,want", [(None, "8"), ("4", "8"), ("8", "8"), ("16", "16")])
+def test_loading_raises_hardware_queues_to_the_slot_count(given, want):
+    """Loading the library raises GPU_MAX_HW_QUEUES to ES_MAX_IN_FLIGHT (8) when it is unset or
+    lower, and leaves a larger value alone (checked in a fresh process, without a GPU call)."""
+    import subprocess
+    import sys
+    code = ("import ctypes; ctypes.CDLL(%r); libc = ctypes.CDLL(None); libc.getenv.restype = ctypes.c_char_p; "
+            "print(libc.getenv(b'GPU_MAX_HW_QUEUES').decode())" % paths.SIEVE_LIB)
+    env = dict(os.environ)
+    env.pop("GPU_MAX_HW_QUEUES", None)
+    if given is not None:
+        env["GPU_MAX_HW_QUEUES"] = given
+    out = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE, check=True, timeout=120)
+    assert out.stdout.decode().strip() == want
