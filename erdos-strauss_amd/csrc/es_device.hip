@@ -572,7 +572,11 @@ __global__ void k_stage(const BatchHdr *__restrict__ h_hdr, const uint2 *__restr
                         u32 *__restrict__ scalars, u64 *__restrict__ board_words, u32 nboard_words,
                         u64 *__restrict__ h_sync, u64 *__restrict__ stamps, u32 *__restrict__ active) {
   const u32 g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  const BatchHdr h = *h_hdr;
+  // The header crosses PCIe once per workgroup, not once per wave.
+  __shared__ BatchHdr sh;
+  if (threadIdx.x == 0) sh = *h_hdr;
+  __syncthreads();
+  const BatchHdr h = sh;
   if (g == 0) {
     *hdr = h;
     h_sync[1] = __builtin_amdgcn_s_memrealtime();   // the batch's start on the device clock
@@ -2128,7 +2132,7 @@ int DeviceCtx::enqueue_batch(int si) {
   Slot &b = m.slot[si];
   const LaunchCfg &cfg = b.cfg;
   auto enqueue = [&](hipStream_t st, bool timed) -> int {
-    dev::k_stage<<<std::max(1, cus_ * 4), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
+    dev::k_stage<<<std::max(1, cus_), 256, 0, st>>>(b.dh_hdr, (const uint2 *)b.dh_tiles, b.d_hdr,
                                                         (uint2 *)b.d_tiles, (const uint4 *)b.dh_items,
                                                         (uint4 *)b.d_items, b.d_tile_counts, b.d_scalars,
                                                         (u64 *)b.d_board, dev::kBoardSlots * 4u, b.dh_sync,
@@ -2168,7 +2172,7 @@ int DeviceCtx::enqueue_batch(int si) {
                             (u32)b.surv_cap, b.d_flags, (const u64 *)b.d_one, b.d_stamps + 4);
     }
     HIP_TRY(hipGetLastError());
-    dev::k_pack<<<std::max(1, cus_), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
+    dev::k_pack<<<std::max(1, cus_ / 4), 256, 0, st>>>(b.d_hdr, b.d_tile_counts, b.d_scalars, (u32)b.surv_cap,
                                                    b.d_surv_p, b.d_surv_w, b.d_flags, b.d_rung,
                                                    cfg.device_tail ? 1u : 0u, b.dh_count, b.dh_tile_counts,
                                                    b.dh_surv_p, b.dh_surv_w, b.dh_flags, b.dh_rung, b.d_scalars + 4,
