@@ -177,16 +177,21 @@ def kernel_split(sc, batch, want, reps=5):
     return out, ok
 
 
-def marking_roofline(sc, batch, sieve_ms, layouts=None):
+def marking_roofline(sc, batch, sieve_ms, layouts=None, wheel=None):
     """Algorithmic LDS bytes of k_sieve over a batch (8 B per 32-bit read-modify-write mark,
-    8 B per 32 bitmap positions for the init and the count) against the nominal LDS rate."""
-    marks = bits = pairs = tiles = 0
+    8 B per 32 bitmap positions for the init and the count) against the nominal LDS rate.
+    `marks` counts the marks the chosen layouts execute; with `wheel`, `marks_lane` counts
+    those of the full-wheel layout (one lane per wheel class), the layout-independent useful
+    marks: a sub-wheel layout also marks positions its periodic wheel mask has cleared."""
+    marks = bits = pairs = tiles = lane_marks = 0
     for lo, hi in batch:
         w = sc.mark_count(lo, hi, 0)
         marks += w["marks"]
         bits += w["bits"]
         pairs += w["pairs"]
         tiles += w["tiles"]
+        if wheel:
+            lane_marks += sc.mark_count(lo, hi, wheel)["marks"]
     lds_bytes = 8 * marks + 8 * (bits // 32)
     achieved = lds_bytes / (sieve_ms * 1e-3) / 1e9
     return {"bound": "lds", "kernel": "k_sieve", "achieved": achieved, "peak": NOMINAL_LDS_BPS / 1e9,
@@ -194,7 +199,8 @@ def marking_roofline(sc, batch, sieve_ms, layouts=None):
             "marks": marks, "bits": bits, "pairs": pairs, "tiles": tiles, "kernel_ms": sieve_ms,
             "kernel_ms_from": "the kernel's own device-clock stamps (first workgroup's start to last "
                               "workgroup's end), median of 5 graph replays one at a time",
-            "marks_per_s": marks / (sieve_ms * 1e-3)}
+            "marks_per_s": marks / (sieve_ms * 1e-3),
+            **({"marks_lane": lane_marks, "marks_lane_over_marks": lane_marks / marks} if wheel and marks else {})}
 
 
 def c4_wheel(es, M, F, steps, warmup):
@@ -213,7 +219,7 @@ def c4_wheel(es, M, F, steps, warmup):
     pipelined(sc, wins, es.MAX_IN_FLIGHT, gold)                 # every slot's graph captured
     wall, ok2 = pipelined(sc, wins, steps, gold)
     split, ok3 = kernel_split(sc, wins, gold)
-    roof = marking_roofline(sc, wins, split["sieve_ms"])
+    roof = marking_roofline(sc, wins, split["sieve_ms"], wheel=M)
     roof["kernel_ms_hip_events_eager"] = split["eager"]["sieve_event_ms"]
     sc.set_graph(False)
     sc.set_launch(False, launch["block"], launch["tile_bits"])
