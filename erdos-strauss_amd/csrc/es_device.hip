@@ -671,8 +671,11 @@ constexpr u8 kComposite = 0, kPrime = 1, kPending = 2;
 // base-2 test. Values that pass are marked pending and keep 2^64 mod n for pass 2.
 __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
                            u8 *__restrict__ flags, u64 *__restrict__ one_out, u64 *stamp = nullptr) {
-  stamp_start(stamp);
   const u32 n_ = min(*count, cap);
+  // Only workgroups with values stamp: a same-address atomic from each of the grid's idle
+  // workgroups would serialize at the memory side and hold the kernel's end back.
+  const bool works = blockIdx.x == 0 || blockIdx.x * blockDim.x < n_;
+  if (works) stamp_start(stamp);
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n_; i += gridDim.x * blockDim.x) {
     const u64 n = vals[i];
     u8 v;
@@ -704,7 +707,7 @@ __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__
     }
     flags[i] = v;
   }
-  if (stamp) {
+  if (stamp && works) {
     __syncthreads();
     stamp_end(stamp, threadIdx.x == 0);
   }
@@ -715,9 +718,10 @@ __global__ void k_mr_base2(const u64 *__restrict__ vals, const u32 *__restrict__
 // values that survive every base are prime.
 __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__ count, u32 cap,
                            u8 *__restrict__ flags, const u64 *__restrict__ one_in, u64 *stamp = nullptr) {
-  stamp_start(stamp);
   const u32 n_ = min(*count, cap);
   const u32 total = n_ * 11u;
+  const bool works = blockIdx.x == 0 || blockIdx.x * blockDim.x < total;   // as in k_mr_base2
+  if (works) stamp_start(stamp);
   for (u32 g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
     const u32 i = g / 11u, b = g - i * 11u + 1u;
     if (flags[i] != kPending) continue;
@@ -736,7 +740,7 @@ __global__ void k_mr_bases(const u64 *__restrict__ vals, const u32 *__restrict__
     }
     if (!ok) flags[i] = kComposite;
   }
-  if (stamp) {
+  if (stamp && works) {
     __syncthreads();
     stamp_end(stamp, threadIdx.x == 0);
   }
