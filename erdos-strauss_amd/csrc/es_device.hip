@@ -308,7 +308,9 @@ __global__ __launch_bounds__(1024) void k_sieve(const TileDesc *__restrict__ til
   const u32 nitems = hdr->nitems;
   const bool sieve = hdr->has_sieve != 0;
   const u32 wave = tid >> 6, nwaves = (bs + 63u) >> 6;
-  stamp_start(stamp);
+  // Workgroups are dispatched in order of their index, so workgroup 0's start is the
+  // kernel's (one atomic instead of one per workgroup at the same instant).
+  if (blockIdx.x == 0) stamp_start(stamp);
   // Persistent form: workgroup g takes items g and gridDim.x + g first, and the queue
   // hands out items from 2 gridDim.x on (every workgroup claiming from one counter at
   // the start would serialize on that address). A workgroup loads the next item's
@@ -1463,7 +1465,7 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
-  stamp_start(stamp);
+  if (blockIdx.x == 0) stamp_start(stamp);   // the first workgroup dispatched, as in k_sieve
   // A persistent grid takes survivors from a queue, so every prime starts as soon as a
   // workgroup is free (the index slot alternates, so the next claim never overwrites an
   // index a wave has yet to read).
