@@ -1446,6 +1446,10 @@ __device__ bool wave_direct_divisor(u64 n, u64 q4, u64 a, const WaveFac &F, u32 
   return true;
 }
 
+// Batches of at most kPendScan survivors are listed: k_rung's queue hands out the values
+// k_mr_base2 left pending (at most kPendList of them) instead of every survivor.
+constexpr u32 kPendScan = 65536, kPendList = 4096;
+
 __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict__ vals,
                                                           const u32 *__restrict__ count, u32 cap,
                                                           u8 *__restrict__ flags, const u64 *__restrict__ one_in,
@@ -1462,18 +1466,54 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
   __shared__ u32 mr_composite;
   __shared__ u32 cur_idx[2];
   __shared__ u32 help_batch;
+  __shared__ u32 pend[kPendList];
+  __shared__ u32 npend, wcnt[kRungWaves];
   const u32 tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const u32 n_ = min(*count, cap);
   const volatile u32 *stop = &stop_k;
   if (blockIdx.x == 0) stamp_start(stamp);   // the first workgroup dispatched, as in k_sieve
+  // Most survivors are composites k_mr_base2 has already marked. Claiming each of them from
+  // the one queue counter (and counting each done) puts hundreds of same-address atomics
+  // ahead of the first prime, and they serialize at the memory side. So in a batch of at
+  // most kPendScan survivors every workgroup lists the pending ones from one pass over the
+  // flags, in index order, so that every workgroup's list is the same and a list position
+  // names the same survivor everywhere; the queue hands out list positions. A larger batch,
+  // or one with more than kPendList pending values, takes survivors from the queue as they
+  // are. Either way `done` counts the items the queue hands out, nq of them.
+  bool listed = n_ <= kPendScan;
+  if (listed) {
+    if (tid == 0) npend = 0;
+    __syncthreads();
+    for (u32 b0 = 0; b0 < n_; b0 += blockDim.x) {
+      const u32 i = b0 + tid;
+      const bool pending = i < n_ && flags[i] != kComposite;
+      const u64 m = __ballot(pending);
+      if (lane == 0) wcnt[wv] = (u32)__popcll(m);
+      __syncthreads();                     // every wave's count, and npend, before any update
+      u32 base = npend, total = 0;
+      for (u32 w = 0; w < kRungWaves; ++w) {
+        if (w < wv) base += wcnt[w];
+        total += wcnt[w];
+      }
+      if (pending) {
+        const u32 at = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
+        if (at < kPendList) pend[at] = i;
+      }
+      __syncthreads();                     // every read of npend and wcnt before they change
+      if (tid == 0) npend += total;
+    }
+    __syncthreads();
+    listed = npend <= kPendList;   // uniform: read after the barrier
+  }
+  const u32 nq = listed ? npend : n_;
   // A persistent grid takes survivors from a queue, so every prime starts as soon as a
   // workgroup is free (the index slot alternates, so the next claim never overwrites an
   // index a wave has yet to read).
   for (u32 it = 0;; ++it) {
     if (tid == 0) cur_idx[it & 1] = atomicAdd(queue, 1u);
     __syncthreads();
-    const u32 idx = cur_idx[it & 1];
-    if (idx >= n_) {
+    const u32 q = cur_idx[it & 1];
+    if (q >= nq) {
       // The queue is empty: help the splits other workgroups post, one wave per SIMD.
       // Only a batch with a prime p >= 2^52 can need a Montgomery split (A = (p + r) / 4
       // is then at least 2^50); below that every split runs in FP64 and is fast alone,
@@ -1486,10 +1526,11 @@ __global__ __launch_bounds__(64 * kRungWaves) void k_rung(const u64 *__restrict_
       if (big) help_batch = 1;             // any writer writes the same value
       __syncthreads();
       if (help_batch && wv < kBoardHelpWaves)
-        board_help(board, done, n_, lane, blockIdx.x * kBoardHelpWaves + wv,
+        board_help(board, done, nq, lane, blockIdx.x * kBoardHelpWaves + wv,
                    blockIdx.x >= kBoardHelpGroups ? active : nullptr);
       break;
     }
+    const u32 idx = listed ? pend[q] : q;
     const u8 f0 = flags[idx];
     if (f0 == kComposite) {
       if (tid == 0 && done) atomicAdd(done, 1u);
