@@ -54,11 +54,18 @@ SIEVE_ORDER = [
 ]
 
 RUNG_PHASES = [
+    ("pending reset", "if (tid == 0) npend = 0;"),
+    ("pending counts", "if (lane == 0) wcnt[wv] = (u32)__popcll(m);"),
+    ("pending bases", "u32 base = npend, total = 0;"),
+    ("pending list", "if (at < kPendList) pend[at] = i;"),
+    ("pending total", "if (tid == 0) npend += total;"),
+    ("list length", "listed = npend <= kPendList;"),
     ("index claim", "cur_idx[it & 1] = atomicAdd(queue, 1u);"),
-    ("index read", "const u32 idx = cur_idx[it & 1];"),
+    ("index read", "const u32 q = cur_idx[it & 1];"),
     ("help gate cleared", "if (tid == 0) help_batch = 0;"),
     ("help gate set", "if (big) help_batch = 1;"),
     ("help gate read", "if (help_batch && wv < kBoardHelpWaves)"),
+    ("list read", "const u32 idx = listed ? pend[q] : q;"),
     ("prime state init", "if (tid < kRungSlots) nfac[tid] = 0;"),
     ("bases and rung sieve", "const bool pending = f0 == kPending;"),
     ("verdict read", "if (mr_composite) {"),
@@ -69,6 +76,11 @@ RUNG_PHASES = [
     ("next prime", "stamp_end(stamp, lane == 0);"),
 ]
 RUNG_ORDER = [
+    ("pending reset", "pending bases"),            # npend
+    ("pending counts", "pending bases"),           # wcnt
+    ("pending bases", "pending total"),            # npend and wcnt read before npend changes
+    ("pending total", "list length"),              # npend
+    ("pending list", "list read"),                 # pend
     ("index claim", "index read"),                 # cur_idx
     ("help gate cleared", "help gate set"),        # help_batch
     ("help gate set", "help gate read"),
